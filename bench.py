@@ -1,0 +1,174 @@
+"""Benchmark of the Ni-Cr phase-field time step on one MI355X.
+
+Workload (default): 256^3 cells, 32 eta fields evolving (64 Voronoi seeds, each
+assigned one of 32 eta fields by colouring, so neighbouring grains differ),
+Cr = 0.20 +- U(0.005) noise, O = 0 with the inlet layer at x = 0, Ksp
+insertion on, --bounds legacy, repo-default mobilities and dt. One step = one
+accepted time step over the whole mesh (mu, mobility/flux/phi trial, eta trial,
+acceptance). Inputs are resident in HBM when the timed region starts.
+
+Prints ONE JSON line: value = mixed-precision Gcell-steps/s; the FP64 step is
+measured on the same state and reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG_NAME = "ni-cr_oxidation_phase-field_simulation_amd"
+HBM_PEAK_GBS = 8000.0
+
+
+def _pkg():
+    if REPO not in sys.path:
+        sys.path.insert(0, REPO)
+    return importlib.import_module(PKG_NAME)
+
+
+def build_state(nicr, shape, n_seeds, n_eta, seed):
+    syn = importlib.import_module(PKG_NAME + ".synthetic")
+    t0 = time.time()
+    labels, dist, _ = syn.voronoi_labels(shape, n_seeds, seed=seed)
+    colour = syn.colour_seeds(labels, n_seeds, n_eta)
+    o, cr, p = syn.noisy_alloy(shape, 0.20, 0.005, seed=seed)
+    eta = syn.eta_from_labels(labels, dist, colour, n_eta)
+    print(f"[bench] state {shape} with {n_seeds} seeds -> {n_eta} eta built in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    return o, cr, p, eta
+
+
+def make_model(nicr, shape, n_eta, precision, state):
+    o, cr, p, eta = state
+    cfg = nicr.make_config(3, nx=shape[0], ny=shape[1], nz=shape[2], grain_evolution=True, n_eta=n_eta,
+                           bc="concentration", bounds="legacy", ck="legacy", nucleation="ksp",
+                           precision=precision)
+    m = nicr.Model(cfg)
+    m.set("o", o)
+    m.set("cr", cr)
+    m.set("p", p)
+    for g in range(n_eta):
+        m.set(g, eta[g])
+    return m
+
+
+def time_steps(m, steps, warmup, label):
+    r = m.try_step(warmup)
+    if r.status != 0:
+        raise RuntimeError(f"{label}: warmup step rejected (status {r.status})")
+    m.sync()
+    m.reset_kernel_times()
+    m.set_timing(True)
+    t0 = time.perf_counter()
+    r = m.try_step(steps)
+    m.sync()
+    t1 = time.perf_counter()
+    m.set_timing(False)
+    if r.status != 0 or r.steps_done != steps:
+        raise RuntimeError(f"{label}: timed step rejected (status {r.status})")
+    return (t1 - t0), m.kernel_times()
+
+
+def cpu_baseline(nicr, n_eta, seed):
+    """The CPU oracle (one thread) on a 64 x 256 x 256 slab of the same workload, 2 steps."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import binding
+    shape = (64, 256, 256)
+    state = build_state(nicr, shape, 16, n_eta, seed)
+    o, cr, p, eta = state
+    cfg = nicr.make_config(3, nx=shape[0], ny=shape[1], nz=shape[2], grain_evolution=True, n_eta=n_eta,
+                           bc="concentration", bounds="legacy", ck="legacy")
+    ora = binding.OracleModel(cfg)
+    ora.set("o", o)
+    ora.set("cr", cr)
+    ora.set("p", p)
+    for g in range(n_eta):
+        ora.set(g, eta[g])
+    del state, eta
+    nsteps = 2
+    t0 = time.perf_counter()
+    done = ora.run(nsteps)
+    t1 = time.perf_counter()
+    cells = shape[0] * shape[1] * shape[2]
+    return {"value": cells * done / (t1 - t0) / 1e9, "unit": "Gcell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle/liboracle.so, {shape[0]}x{shape[1]}x{shape[2]} cells, {n_eta} eta, "
+                      f"{done} FP64 steps, {t1 - t0:.1f} s on one host thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256, help="mesh side")
+    ap.add_argument("--eta", type=int, default=32)
+    ap.add_argument("--seeds", type=int, default=64)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the X-slab sharding design is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    nicr = _pkg()
+    shape = (args.n, args.n, args.n)
+    cells = shape[0] * shape[1] * shape[2]
+    state = build_state(nicr, shape, args.seeds, args.eta, 20260909)
+
+    m = make_model(nicr, shape, args.eta, "mixed", state)
+    print(f"[bench] mixed model resident: {m.device_bytes() / 2**30:.2f} GiB", file=sys.stderr, flush=True)
+    wall, kt = time_steps(m, args.steps, args.warmup, "mixed")
+    m.close()
+    ms_per_step = wall / args.steps * 1e3
+    value = cells * args.steps / wall / 1e9
+
+    fp64 = None
+    if not args.no_fp64:
+        m64 = make_model(nicr, shape, args.eta, "fp64", state)
+        wall64, kt64 = time_steps(m64, max(10, args.steps // 4), 2, "fp64")
+        n64 = max(10, args.steps // 4)
+        m64.close()
+        fp64 = {"value": cells * n64 / wall64 / 1e9, "ms_per_step": wall64 / n64 * 1e3,
+                "kernel_ms": {k: v[0] for k, v in kt64.items()}}
+    del state
+
+    # roofline of the dominant kernel (k_eta): algorithmic bytes = each eta read once and written once
+    eta_ms = kt.get("k_eta", (float("nan"), 0))[0]
+    eta_bytes = cells * args.eta * 4 * 2
+    achieved = eta_bytes / (eta_ms * 1e-3) / 1e9
+    step_bytes = cells * (48 + 8 * args.eta)   # FP64 o/cr/phi + FP32 eta, read + write
+    out = {
+        "metric": "Gcell-steps/s @256^3 mixed vs FP64 GPU, % HBM peak",
+        "value": round(value, 4),
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64 (mixed); eta fp32",
+        "data": "synthetic",
+        "config": {"workload": f"3D {shape[0]}^3, {args.eta} evolving eta from {args.seeds} Voronoi seeds, "
+                               "Cr 0.20+-0.005, O inlet layer, Ksp on, bounds legacy, mixed precision",
+                   "cells": cells, "n_eta": args.eta},
+        "step_hbm_fraction": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel_ms": {k: round(v[0], 5) for k, v in kt.items()},
+        "roofline": {"bound": "hbm", "kernel": "k_eta", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "fp64": fp64,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(nicr, args.eta, 20260909)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,177 @@
+/* nicr.h -- C ABI of the MI355X Ni-Cr phase-field time-step library (libnicr.so).
+ *
+ * The reference solvers have no plugin or FFI layer: their in-process interface
+ * is `class Model` (mpi2d/NiCr2D_grains.cpp:194-711, mpi3d-slab/NiCr3D_grains.cpp:336-978).
+ * Every entry point below replaces one Model member or public field; the citation
+ * on each declaration names the member it stands in for.
+ *
+ * Conventions
+ *  - Plain C, plain pointers and sizes. No torch or HIP types cross this boundary.
+ *  - Host arrays hold INTERIOR cells only, in the reference's storage order without
+ *    halos: 3D index ((i-1)*ny + (j-1))*nz + (k-1) (Z fastest, mpi3d-slab:165-167);
+ *    2D index (i-1)*ny + (j-1) (Y fastest, mpi2d:149). Values are IEEE binary64.
+ *  - Device buffers belong to the context; host pointers are borrowed for one call.
+ *  - One context per host thread; calls on one context are serialized; no callbacks.
+ *  - Every function returns NICR_OK (0) or a negative NICR_E_* code; the message
+ *    is then available from nicr_last_error().
+ */
+#ifndef NICR_H
+#define NICR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NICR_ABI_VERSION 1
+
+/* Return codes. */
+#define NICR_OK 0
+#define NICR_E_INVALID (-1)   /* bad argument or configuration */
+#define NICR_E_DEVICE (-2)    /* HIP runtime failure */
+#define NICR_E_STATE (-3)     /* call not valid in the current state */
+
+/* Step status (nicr_step_report.status). Mirrors the three runtime_error texts
+ * thrown by Model::step (mpi2d:573, mpi2d:585-588, mpi2d:352-354). */
+#define NICR_STEP_OK 0
+#define NICR_STEP_NONFINITE 1      /* "Nonfinite trial field ..." */
+#define NICR_STEP_OUT_OF_BOUNDS 2  /* "Out-of-bounds trial at (...)" (bounds=stop) */
+#define NICR_STEP_ETA_RANGE 3      /* "Eta trial outside [0,1] ..." */
+
+/* Enumerations of the reference's string options (mpi2d:48-50). */
+#define NICR_BC_CONCENTRATION 0
+#define NICR_BC_MU 1
+#define NICR_BC_CLOSED 2
+#define NICR_CK_LEGACY 0
+#define NICR_CK_OFF 1
+#define NICR_BOUNDS_STOP 0
+#define NICR_BOUNDS_LEGACY 1
+#define NICR_NUC_KSP 0
+#define NICR_NUC_CENTER 1
+#define NICR_NUC_OFF 2
+
+/* Arithmetic of the device step. FP64 reproduces the reference's binary64
+ * operation sequence; MIXED evaluates stencils and chemical potentials in
+ * binary32 and stores eta in binary32, while O/Cr/phi state, thresholds, the
+ * inlet flux and all budgets stay binary64. */
+#define NICR_PRECISION_FP64 0
+#define NICR_PRECISION_MIXED 1
+
+/* Field identifiers for upload/download. */
+#define NICR_FIELD_O 0        /* Model::o */
+#define NICR_FIELD_CR 1       /* Model::cr */
+#define NICR_FIELD_PHI 2      /* Model::p */
+#define NICR_FIELD_GB 3       /* Model::gb (static mask, or eta-derived after refresh) */
+#define NICR_FIELD_MU_O 4     /* Model::muO  (valid after nicr_refresh) */
+#define NICR_FIELD_MU_CR 5    /* Model::muCr (valid after nicr_refresh) */
+#define NICR_FIELD_M_O 6      /* Model::mo   (valid after nicr_refresh) */
+#define NICR_FIELD_M_CR 7     /* Model::mc   (valid after nicr_refresh) */
+#define NICR_FIELD_ETA_S2 8   /* Model::eta_s2 (sum of eta^2; eta mode) */
+#define NICR_FIELD_ETA_PAIRS 9 /* mpi2d Model::eta_pairs (sum over pairs of eta_a^2 eta_b^2) */
+#define NICR_FIELD_ETA0 64    /* NICR_FIELD_ETA0 + g is eta field g (0-based) */
+
+/* Mirror of the reference Config (mpi2d:39-64, mpi3d-slab:43-72) as a POD.
+ * `dim` selects whose step semantics apply: 2 = mpi2d, 3 = mpi3d-slab. */
+typedef struct nicr_config {
+    int32_t abi_version;      /* must be NICR_ABI_VERSION */
+    int32_t dim;              /* 2 or 3 */
+    int32_t nx, ny, nz;       /* nz is ignored (taken as 1) when dim == 2 */
+    int32_t steps, output_every, left_edge_sites;
+    uint64_t random_seed;
+    double dx, dy, dz, dt;
+    double grain_diameter_grid, gb_fwhm_grid;
+    double gb_factor_o, gb_factor_cr;
+    double radius_grid, exclusion_gap_grid, nucleation_period;
+    double mu_res, surface_c_ref;
+    int32_t bc, ck, bounds, nucleation;
+    int32_t grains, oxygen_diffusion, grain_evolution, frozen_eta_cache;
+    double eta_W, eta_kappa, eta_L, eta_oxide_penalty;
+    int32_t max_eta_fields;
+    int32_t n_eta;            /* eta fields held (0 unless grains && grain_evolution) */
+    int32_t precision;        /* NICR_PRECISION_* */
+    int32_t reserved0;
+    /* Derived eta coefficients (mpi2d:88-103); filled by nicr_config_derive(). */
+    double eta_K, eta_A, eta_scale;
+} nicr_config;
+
+/* Budget (mpi2d:193, mpi3d-slab:217). The reference keeps long double sums; the
+ * device keeps each as an unevaluated double-double (hi + lo). */
+typedef struct nicr_budget {
+    double nucO[2], nucCr[2], ckO[2], ckCr[2], clipO[2], clipCr[2], bcO[2], bcCr[2];
+    int64_t nuclei, clipped;
+} nicr_budget;
+
+/* Outcome of nicr_step. On failure the device state equals the state after the
+ * last accepted step, and budgets are those of that step (mpi3d-slab:811-820). */
+typedef struct nicr_step_report {
+    int32_t status;           /* NICR_STEP_* */
+    int32_t steps_done;       /* accepted steps in this call */
+    int32_t i, j, k;          /* 1-based first failing cell in the reference's loop order */
+    int32_t grain;            /* 1-based eta index for NICR_STEP_ETA_RANGE */
+    double o, cr, phi;        /* trial values at the failing cell (OUT_OF_BOUNDS) */
+    double eta_value;         /* trial eta value (ETA_RANGE) */
+} nicr_step_report;
+
+/* Reduction of one field over the interior (Model::sum, mpi2d:229-232; log()
+ * extrema, mpi2d:702-705). sum is a double-double. */
+typedef struct nicr_stats {
+    double sum[2];
+    double min, max;
+} nicr_stats;
+
+/* ---- configuration ---- */
+/* Fill a config with the reference defaults (Config{} at mpi2d:39-64 for dim 2,
+ * mpi3d-slab:43-72 for dim 3). */
+int nicr_config_default(nicr_config* cfg, int dim);
+/* Compute eta_K, eta_A, eta_scale (eta_K, eta_A, eta_transport_scale: mpi2d:88-103). */
+int nicr_config_derive(nicr_config* cfg);
+
+/* ---- lifecycle: replaces Model::Model's storage (mpi2d:209-228, mpi3d-slab:356-381) ---- */
+int nicr_create(const nicr_config* cfg, void** ctx);
+int nicr_destroy(void* ctx);
+const char* nicr_last_error(void);
+/* Device bytes held by the context. */
+int nicr_device_bytes(void* ctx, size_t* bytes);
+
+/* ---- fields: replaces direct access to Model::o/cr/p/gb/eta (public Field members) ---- */
+int nicr_upload_field(void* ctx, int field, const double* host);
+int nicr_download_field(void* ctx, int field, double* host);
+/* Replaces assignment to Model::b and Model::initialO/Cr. */
+int nicr_set_budget(void* ctx, const nicr_budget* b);
+int nicr_get_budget(void* ctx, nicr_budget* b);
+
+/* ---- time stepping ---- */
+/* Model::refresh (mpi2d:412-434, mpi3d-slab:597-620): eta mask, mu and M arrays. */
+int nicr_refresh(void* ctx);
+/* nsteps x Model::step (mpi2d:551-597, mpi3d-slab:761-826) with all-or-nothing
+ * acceptance. Stops at the first rejected step. */
+int nicr_step(void* ctx, int nsteps, nicr_step_report* report);
+
+/* ---- oxide insertion ---- */
+/* Model::nucleate (mpi2d:543-550, mpi3d-slab:752-759): ordered Ksp scan. */
+int nicr_nucleate(void* ctx, int* inserted_cells);
+/* Model::stamp (mpi2d:503-516, mpi3d-slab:704-724); coordinates are 1-based. */
+int nicr_stamp(void* ctx, int x, int y, int z, int* changed);
+
+/* ---- diagnostics ---- */
+/* Model::sum plus extrema of one interior field. */
+int nicr_field_stats(void* ctx, int field, nicr_stats* out);
+/* Sum over the interior of h(phi) (mpi3d-slab:970). */
+int nicr_sum_h_phi(void* ctx, double out_sum[2]);
+
+/* ---- timing support ---- */
+/* Mean device duration (ms) of each step kernel over the launches since the last
+ * reset, measured with HIP events on the context's stream. names[] gets up to
+ * max entries "kernel=ms". Returns the number of entries. */
+int nicr_kernel_times(void* ctx, char* buf, size_t buflen);
+int nicr_reset_kernel_times(void* ctx);
+int nicr_set_timing(void* ctx, int enabled);
+/* Block until the device has finished all work queued on the context. */
+int nicr_sync(void* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NICR_H */

@@ -1,0 +1,810 @@
+// nicr_api.hip -- C ABI of libnicr.so (declared in include/nicr.h).
+//
+// Owns the device state of one simulation (the storage of class Model,
+// mpi2d/NiCr2D_grains.cpp:194-207, mpi3d-slab/NiCr3D_grains.cpp:336-355) and
+// queues the step kernels of nicr_step.hip on the context's own HIP stream.
+#include "../../include/nicr.h"
+#include "nicr_kernels.hpp"
+#include "nicr_step.hip"
+#include "nicr_nucleate.hip"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace nicr;
+
+namespace {
+
+thread_local std::string g_error;
+constexpr double PI = 3.14159265358979323846;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(NICR_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+struct TimerSlot {
+    const char* name;
+    double total_ms = 0.;
+    long long count = 0;
+};
+
+struct Ctx {
+    nicr_config cfg{};
+    Par P{};
+    int dim = 3, prec = PREC_FP64;
+    long long ncell = 0;
+    size_t te = 8, tm = 8;  // element sizes of eta/mask and mu scratch
+    hipStream_t stream = nullptr;
+    double* o[2] = {nullptr, nullptr};
+    double* cr[2] = {nullptr, nullptr};
+    double* p[2] = {nullptr, nullptr};
+    void* mu_o = nullptr;
+    void* mu_cr = nullptr;
+    void* eta[2] = {nullptr, nullptr};
+    void* s2[2] = {nullptr, nullptr};
+    void* pairs[2] = {nullptr, nullptr};
+    void* gb[2] = {nullptr, nullptr};
+    double* partials = nullptr;
+    DevCtl* ctl = nullptr;
+    double* stat_part = nullptr;
+    double* out4[4] = {nullptr, nullptr, nullptr, nullptr};  // refresh outputs muO, muCr, mo, mc
+    double* dev_obuf = nullptr;  // device copy of {o0, o1, cr0, cr1, p0, p1}
+    int* nuc_tiles = nullptr;    // nucleation scratch (allocated on first use)
+    int* nuc_planes = nullptr;
+    int* nuc_list = nullptr;
+    int* nuc_misc = nullptr;     // [0] candidate count, [1] changed/cells out
+    int* nuc_centres = nullptr;
+    int g_trial = 0, g_eta = 0, g_stat = 0;
+    int cur = 0;                 // host mirror of ctl->cur
+    bool mask_dirty = true;
+    bool refresh_valid = false;
+    size_t bytes = 0;
+    std::vector<void*> allocs;
+    bool timing = false;
+    TimerSlot timers[4] = {{"k_mu"}, {"k_trial"}, {"k_eta"}, {"k_commit"}};
+    std::vector<hipEvent_t> evpool;
+    std::vector<std::pair<int, int>> pending;  // (slot, event index of start)
+    size_t evnext = 0;
+
+    bool has_eta() const { return cfg.grains && cfg.grain_evolution && cfg.n_eta > 0; }
+    bool eta_evolves() const { return has_eta() && (dim == 2 || cfg.eta_L > 0.); }
+};
+
+int dalloc(Ctx* c, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(NICR_E_DEVICE, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    c->allocs.push_back(*p);
+    c->bytes += bytes;
+    return NICR_OK;
+}
+
+template <class TE, class TM>
+Bufs<TE, TM> bufs(Ctx* c) {
+    Bufs<TE, TM> b;
+    for (int i = 0; i < 2; ++i) {
+        b.o[i] = c->o[i];
+        b.cr[i] = c->cr[i];
+        b.p[i] = c->p[i];
+        b.eta[i] = (TE*)c->eta[i];
+        b.s2[i] = (TE*)c->s2[i];
+        b.pairs[i] = (TE*)c->pairs[i];
+        b.gb[i] = (TE*)c->gb[i];
+    }
+    b.mu_o = (TM*)c->mu_o;
+    b.mu_cr = (TM*)c->mu_cr;
+    b.partials = c->partials;
+    b.ctl = c->ctl;
+    return b;
+}
+
+// ---- timing with HIP events on the context stream ----
+void tmark_begin(Ctx* c, int slot) {
+    if (!c->timing) return;
+    if (c->evnext + 2 > c->evpool.size()) {
+        for (int i = 0; i < 64; ++i) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) == hipSuccess) c->evpool.push_back(e);
+        }
+    }
+    hipEventRecord(c->evpool[c->evnext], c->stream);
+    c->pending.push_back({slot, (int)c->evnext});
+    c->evnext += 2;
+}
+void tmark_end(Ctx* c) {
+    if (!c->timing) return;
+    hipEventRecord(c->evpool[c->pending.back().second + 1], c->stream);
+}
+void tcollect(Ctx* c) {
+    if (!c->timing) return;
+    for (auto& pr : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->evpool[pr.second], c->evpool[pr.second + 1]) == hipSuccess) {
+            c->timers[pr.first].total_ms += ms;
+            c->timers[pr.first].count += 1;
+        }
+    }
+    c->pending.clear();
+    c->evnext = 0;
+}
+
+// ---- launches, dispatched over (DIM, PREC) ----
+template <int DIM, int PREC, class TE, class TM>
+void launch_mask(Ctx* c) {
+    hipLaunchKernelGGL((k_mask<DIM, PREC, TE, TM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P,
+                       bufs<TE, TM>(c), c->eta_evolves() ? -1 : 0);
+}
+template <int DIM, int PREC, class TE, class TM>
+void launch_step(Ctx* c) {
+    auto B = bufs<TE, TM>(c);
+    tmark_begin(c, 0);
+    hipLaunchKernelGGL((k_mu<DIM, PREC, TE, TM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P, B);
+    tmark_end(c);
+    tmark_begin(c, 1);
+    hipLaunchKernelGGL((k_trial<DIM, PREC, TE, TM>), dim3(c->g_trial), dim3(TILE), 0, c->stream, c->P, B);
+    tmark_end(c);
+    if (c->eta_evolves()) {
+        tmark_begin(c, 2);
+        hipLaunchKernelGGL((k_eta<DIM, PREC, TE, TM>), dim3(c->g_eta), dim3(TILE), 0, c->stream, c->P, B);
+        tmark_end(c);
+    }
+    tmark_begin(c, 3);
+    hipLaunchKernelGGL(k_commit, dim3(1), dim3(1024), 0, c->stream, bufs<double, double>(c), c->g_trial);
+    tmark_end(c);
+}
+template <int DIM, class TE, class TM>
+void launch_refresh_out(Ctx* c) {
+    hipLaunchKernelGGL((k_refresh_out<DIM, TE, TM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P,
+                       bufs<TE, TM>(c), c->out4[0], c->out4[1], c->out4[2], c->out4[3]);
+}
+
+int ensure_nuc_scratch(Ctx* c) {
+    if (c->nuc_list) return NICR_OK;
+    int rc = dalloc(c, (void**)&c->nuc_tiles, (size_t)c->P.ntiles * sizeof(int));
+    if (!rc) rc = dalloc(c, (void**)&c->nuc_planes, (size_t)c->P.nx * sizeof(int));
+    if (!rc) rc = dalloc(c, (void**)&c->nuc_list, (size_t)c->ncell * sizeof(int));
+    if (!rc) rc = dalloc(c, (void**)&c->nuc_misc, 4 * sizeof(int));
+    if (!rc) rc = dalloc(c, (void**)&c->nuc_centres, 4096 * sizeof(int));
+    return rc;
+}
+
+template <int DIM, class TE>
+void launch_stamp(Ctx* c, NucPar N) {
+    double* const* d = (double* const*)c->dev_obuf;
+    TE* eta = c->has_eta() ? (TE*)c->eta[c->eta_evolves() ? c->cur : 0] : nullptr;
+    hipLaunchKernelGGL((k_stamp<DIM, TE>), dim3(1), dim3(1024), 0, c->stream, c->P, N, d, d + 2, d + 4, eta,
+                       c->ctl, c->nuc_misc + 1);
+}
+template <int DIM, class TE>
+void launch_nucleate(Ctx* c, NucPar N) {
+    double* const* d = (double* const*)c->dev_obuf;
+    TE* eta = c->has_eta() ? (TE*)c->eta[c->eta_evolves() ? c->cur : 0] : nullptr;
+    hipMemsetAsync(c->nuc_planes, 0, (size_t)c->P.nx * sizeof(int), c->stream);
+    hipLaunchKernelGGL((k_ksp_mark<DIM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P, N.r, d, d + 2, d + 4,
+                       (const DevCtl*)c->ctl, c->nuc_tiles, c->nuc_planes);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c->stream, c->nuc_tiles, c->P.ntiles, c->nuc_misc);
+    hipLaunchKernelGGL((k_ksp_compact<DIM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P, N.r, d, d + 2,
+                       d + 4, (const DevCtl*)c->ctl, (const int*)c->nuc_tiles, c->nuc_list);
+    hipLaunchKernelGGL((k_ksp_resolve<DIM, TE>), dim3(1), dim3(1024), 0, c->stream, c->P, N, d, d + 2, d + 4, eta,
+                       c->ctl, (const int*)c->nuc_list, (const int*)c->nuc_misc, (const int*)c->nuc_planes,
+                       c->nuc_centres, c->nuc_misc + 1);
+}
+
+void do_mask(Ctx* c) {
+    if (!c->has_eta()) return;
+    if (c->prec == PREC_FP64) {
+        if (c->dim == 2) launch_mask<2, PREC_FP64, double, double>(c);
+        else launch_mask<3, PREC_FP64, double, double>(c);
+    } else {
+        if (c->dim == 2) launch_mask<2, PREC_MIXED, float, float>(c);
+        else launch_mask<3, PREC_MIXED, float, float>(c);
+    }
+}
+void do_step(Ctx* c) {
+    if (c->prec == PREC_FP64) {
+        if (c->dim == 2) launch_step<2, PREC_FP64, double, double>(c);
+        else launch_step<3, PREC_FP64, double, double>(c);
+    } else {
+        if (c->dim == 2) launch_step<2, PREC_MIXED, float, float>(c);
+        else launch_step<3, PREC_MIXED, float, float>(c);
+    }
+}
+void do_refresh_out(Ctx* c) {
+    if (c->prec == PREC_FP64) {
+        if (c->dim == 2) launch_refresh_out<2, double, double>(c);
+        else launch_refresh_out<3, double, double>(c);
+    } else {
+        if (c->dim == 2) launch_refresh_out<2, float, float>(c);
+        else launch_refresh_out<3, float, float>(c);
+    }
+}
+
+__global__ void k_reset_ctl(DevCtl* ctl) {
+    ctl->failed = 0;
+    ctl->steps_done = 0;
+    ctl->fail_kind = 0;
+    ctl->pde_key = NO_KEY;
+    ctl->eta_key = NO_KEY;
+}
+
+int sync_ctl(Ctx* c, DevCtl* h) {
+    HIPCHK(hipMemcpyAsync(h, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->cur = h->cur;
+    return NICR_OK;
+}
+
+void* eta_ptr(Ctx* c, int which, int g) {
+    return (char*)c->eta[which] + (size_t)g * c->ncell * c->te;
+}
+int mask_index(Ctx* c) { return c->eta_evolves() ? c->cur : 0; }
+
+}  // namespace
+
+extern "C" {
+
+const char* nicr_last_error(void) { return g_error.c_str(); }
+
+int nicr_config_default(nicr_config* cfg, int dim) {
+    if (!cfg || (dim != 2 && dim != 3)) return fail(NICR_E_INVALID, "nicr_config_default: dim must be 2 or 3");
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->abi_version = NICR_ABI_VERSION;
+    cfg->dim = dim;
+    cfg->nx = 151;
+    cfg->ny = 101;
+    cfg->nz = dim == 3 ? 101 : 1;
+    cfg->steps = 0;
+    cfg->output_every = 5000;
+    cfg->left_edge_sites = 0;
+    cfg->random_seed = 20260909ull;
+    cfg->dx = cfg->dy = cfg->dz = 1.e-8;
+    cfg->dt = 2e-10;
+    cfg->grain_diameter_grid = 50.;
+    cfg->gb_fwhm_grid = 4.;
+    cfg->gb_factor_o = 28.2;
+    cfg->gb_factor_cr = 563.;
+    cfg->radius_grid = 5.5;
+    cfg->exclusion_gap_grid = 6.;
+    cfg->nucleation_period = 1.e-7;
+    cfg->mu_res = 364688.;
+    cfg->surface_c_ref = 0.0023;
+    cfg->bc = NICR_BC_CONCENTRATION;
+    cfg->ck = NICR_CK_LEGACY;
+    cfg->bounds = NICR_BOUNDS_STOP;
+    cfg->nucleation = NICR_NUC_KSP;
+    cfg->grains = 1;
+    cfg->oxygen_diffusion = 1;
+    cfg->grain_evolution = 0;
+    cfg->frozen_eta_cache = 1;
+    cfg->eta_W = 1.e4;
+    cfg->eta_kappa = 0.;
+    cfg->eta_L = 0.2;
+    cfg->eta_oxide_penalty = -1.;
+    cfg->max_eta_fields = 512;
+    cfg->n_eta = 0;
+    cfg->precision = NICR_PRECISION_FP64;
+    return nicr_config_derive(cfg);
+}
+
+// eta_K, eta_A, eta_transport_scale (2D:88-103, SL:88-103)
+int nicr_config_derive(nicr_config* c) {
+    if (!c) return fail(NICR_E_INVALID, "null config");
+    const double ref_w = c->gb_fwhm_grid * c->dx * std::sqrt(PI) / (2. * std::sqrt(std::log(2.)));
+    double K;
+    if (c->eta_kappa > 0.) K = c->eta_kappa;
+    else {
+        const double ell = 3. * ref_w / 8.;
+        K = 2. * c->eta_W * ell * ell;
+    }
+    const double ell = std::sqrt(K / (2. * c->eta_W));
+    c->eta_K = K;
+    c->eta_A = c->eta_oxide_penalty < 0. ? c->eta_W : c->eta_oxide_penalty;
+    c->eta_scale = ref_w / (8. * ell / 3.);
+    return NICR_OK;
+}
+
+int nicr_create(const nicr_config* cfg, void** out) {
+    if (!cfg || !out) return fail(NICR_E_INVALID, "nicr_create: null argument");
+    if (cfg->abi_version != NICR_ABI_VERSION) return fail(NICR_E_INVALID, "nicr_create: ABI version mismatch");
+    if (cfg->dim != 2 && cfg->dim != 3) return fail(NICR_E_INVALID, "nicr_create: dim must be 2 or 3");
+    if (cfg->nx < 1 || cfg->ny < 1 || (cfg->dim == 3 && cfg->nz < 1))
+        return fail(NICR_E_INVALID, "nicr_create: mesh dimensions must be positive");
+    if (cfg->precision != NICR_PRECISION_FP64 && cfg->precision != NICR_PRECISION_MIXED)
+        return fail(NICR_E_INVALID, "nicr_create: unknown precision");
+    if (cfg->n_eta < 0 || (cfg->grains && cfg->grain_evolution && cfg->n_eta == 0 && cfg->max_eta_fields < 0))
+        return fail(NICR_E_INVALID, "nicr_create: bad n_eta");
+    const long long nyz = (long long)cfg->ny * (cfg->dim == 3 ? cfg->nz : 1);
+    if (nyz > (1ll << 30) || (long long)cfg->nx * ((nyz + TILE - 1) / TILE) > (1ll << 30))
+        return fail(NICR_E_INVALID, "nicr_create: mesh too large for 32-bit tile indexing");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(NICR_E_DEVICE, "nicr_create: no HIP device is available");
+
+    Ctx* c = new Ctx;
+    c->cfg = *cfg;
+    if (c->cfg.dim == 2) c->cfg.nz = 1;
+    if (!(c->cfg.grains && c->cfg.grain_evolution)) c->cfg.n_eta = 0;
+    c->dim = c->cfg.dim;
+    c->prec = c->cfg.precision;
+    c->te = c->prec == PREC_FP64 ? 8 : 4;
+    c->tm = c->te;
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nz = c->cfg.nz;
+    c->ncell = (long long)nx * ny * nz;
+    Par& P = c->P;
+    P.nx = nx;
+    P.ny = ny;
+    P.nz = nz;
+    P.nyz = ny * nz;
+    P.nchunk = (P.nyz + TILE - 1) / TILE;
+    P.ntiles = nx * P.nchunk;
+    P.ncell = c->ncell;
+    P.n_eta = c->cfg.n_eta;
+    P.bc = c->cfg.bc;
+    P.ck_legacy = c->cfg.ck == NICR_CK_LEGACY;
+    P.bounds_legacy = c->cfg.bounds == NICR_BOUNDS_LEGACY;
+    P.grains = c->cfg.grains;
+    P.oxy = c->cfg.oxygen_diffusion;
+    P.has_eta = c->has_eta();
+    P.eta_evolves = c->eta_evolves();
+    P.enforce_layer = c->cfg.bc == NICR_BC_CONCENTRATION && c->cfg.oxygen_diffusion;
+    P.dx = c->cfg.dx;
+    P.dy = c->cfg.dy;
+    P.dz = c->cfg.dz;
+    P.dt = c->cfg.dt;
+    P.gbfo = c->cfg.gb_factor_o;
+    P.gbfc = c->cfg.gb_factor_cr;
+    P.mu_res = c->cfg.mu_res;
+    P.c_ref = c->cfg.surface_c_ref;
+    P.etaW = c->cfg.eta_W;
+    P.etaK = c->cfg.eta_K;
+    P.etaL = c->cfg.eta_L;
+    P.etaA = c->cfg.eta_A;
+    P.eta_scale = c->cfg.eta_scale;
+    P.fdx2i = (float)(1. / (P.dx * P.dx));
+    P.fdy2i = (float)(1. / (P.dy * P.dy));
+    P.fdz2i = (float)(1. / (P.dz * P.dz));
+    P.fdxi = (float)(1. / P.dx);
+    P.fdyi = (float)(1. / P.dy);
+    P.fdzi = (float)(1. / P.dz);
+
+    int rc = NICR_OK;
+    auto cleanup = [&]() {
+        for (void* p : c->allocs) hipFree(p);
+        if (c->stream) hipStreamDestroy(c->stream);
+        delete c;
+    };
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        cleanup();
+        return fail(NICR_E_DEVICE, "nicr_create: hipStreamCreate failed");
+    }
+    const size_t n8 = (size_t)c->ncell * 8, nte = (size_t)c->ncell * c->te;
+    for (int i = 0; i < 2 && rc == NICR_OK; ++i) {
+        rc = dalloc(c, (void**)&c->o[i], n8);
+        if (rc == NICR_OK) rc = dalloc(c, (void**)&c->cr[i], n8);
+        if (rc == NICR_OK) rc = dalloc(c, (void**)&c->p[i], n8);
+    }
+    if (rc == NICR_OK) rc = dalloc(c, &c->mu_o, (size_t)c->ncell * c->tm);
+    if (rc == NICR_OK) rc = dalloc(c, &c->mu_cr, (size_t)c->ncell * c->tm);
+    const int nbuf = c->eta_evolves() ? 2 : 1;
+    for (int i = 0; i < nbuf && rc == NICR_OK; ++i) {
+        if (c->has_eta()) {
+            rc = dalloc(c, &c->eta[i], nte * c->cfg.n_eta);
+            if (rc == NICR_OK) rc = dalloc(c, &c->s2[i], nte);
+            if (rc == NICR_OK) rc = dalloc(c, &c->pairs[i], nte);
+        }
+        if (rc == NICR_OK) rc = dalloc(c, &c->gb[i], nte);
+    }
+    if (nbuf == 1) {
+        c->eta[1] = c->eta[0];
+        c->s2[1] = c->s2[0];
+        c->pairs[1] = c->pairs[0];
+        c->gb[1] = c->gb[0];
+    }
+    // persistent grids: a multiple of 8 blocks (one run per XCD)
+    c->g_trial = std::min(P.ntiles, 2048);
+    if (c->g_trial > 8) c->g_trial &= ~7;
+    c->g_eta = std::min(P.ntiles, 2048);
+    if (c->g_eta > 8) c->g_eta &= ~7;
+    c->g_stat = 512;
+    if (rc == NICR_OK) rc = dalloc(c, (void**)&c->partials, (size_t)c->g_trial * NPART * 8);
+    if (rc == NICR_OK) rc = dalloc(c, (void**)&c->ctl, sizeof(DevCtl));
+    if (rc == NICR_OK) rc = dalloc(c, (void**)&c->stat_part, (size_t)c->g_stat * 4 * 8);
+    if (rc == NICR_OK) rc = dalloc(c, (void**)&c->dev_obuf, 6 * sizeof(double*));
+    if (rc != NICR_OK) {
+        std::string m = g_error;
+        cleanup();
+        return fail(NICR_E_DEVICE, m);
+    }
+    // zero state, control block
+    for (void* p : c->allocs) (void)p;
+    for (int i = 0; i < 2; ++i) {
+        hipMemsetAsync(c->o[i], 0, n8, c->stream);
+        hipMemsetAsync(c->cr[i], 0, n8, c->stream);
+        hipMemsetAsync(c->p[i], 0, n8, c->stream);
+    }
+    for (int i = 0; i < nbuf; ++i) {
+        if (c->has_eta()) {
+            hipMemsetAsync(c->eta[i], 0, nte * c->cfg.n_eta, c->stream);
+            hipMemsetAsync(c->s2[i], 0, nte, c->stream);
+            hipMemsetAsync(c->pairs[i], 0, nte, c->stream);
+        }
+        hipMemsetAsync(c->gb[i], 0, nte, c->stream);
+    }
+    DevCtl h{};
+    h.cur = 0;
+    h.pde_key = NO_KEY;
+    h.eta_key = NO_KEY;
+    hipMemcpyAsync(c->ctl, &h, sizeof(h), hipMemcpyHostToDevice, c->stream);
+    double* ob[6] = {c->o[0], c->o[1], c->cr[0], c->cr[1], c->p[0], c->p[1]};
+    hipMemcpyAsync(c->dev_obuf, ob, sizeof(ob), hipMemcpyHostToDevice, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        cleanup();
+        return fail(NICR_E_DEVICE, "nicr_create: initialisation failed");
+    }
+    *out = c;
+    return NICR_OK;
+}
+
+int nicr_destroy(void* h) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c) return NICR_OK;
+    hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) hipFree(p);
+    for (hipEvent_t e : c->evpool) hipEventDestroy(e);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return NICR_OK;
+}
+
+int nicr_device_bytes(void* h, size_t* bytes) {
+    if (!h || !bytes) return fail(NICR_E_INVALID, "null argument");
+    *bytes = static_cast<Ctx*>(h)->bytes;
+    return NICR_OK;
+}
+
+// Host layout == device layout (interior cells, reference storage order).
+int nicr_upload_field(void* h, int field, const double* src) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c || !src) return fail(NICR_E_INVALID, "nicr_upload_field: null argument");
+    const size_t n8 = (size_t)c->ncell * 8;
+    void* dst = nullptr;
+    bool convert = false;
+    switch (field) {
+    case NICR_FIELD_O: dst = c->o[c->cur]; break;
+    case NICR_FIELD_CR: dst = c->cr[c->cur]; break;
+    case NICR_FIELD_PHI: dst = c->p[c->cur]; break;
+    case NICR_FIELD_GB:
+        dst = c->gb[mask_index(c)];
+        convert = c->te == 4;
+        if (c->has_eta()) c->mask_dirty = true;
+        break;
+    default:
+        if (field >= NICR_FIELD_ETA0 && field < NICR_FIELD_ETA0 + c->cfg.n_eta) {
+            dst = eta_ptr(c, mask_index(c), field - NICR_FIELD_ETA0);
+            convert = c->te == 4;
+            c->mask_dirty = true;
+        } else {
+            return fail(NICR_E_INVALID, "nicr_upload_field: field cannot be uploaded");
+        }
+    }
+    c->refresh_valid = false;
+    if (convert) {
+        std::vector<float> tmp((size_t)c->ncell);
+        for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)src[i];
+        HIPCHK(hipMemcpyAsync(dst, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+        HIPCHK(hipMemcpyAsync(dst, src, n8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return NICR_OK;
+}
+
+int nicr_download_field(void* h, int field, double* dstp) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c || !dstp) return fail(NICR_E_INVALID, "nicr_download_field: null argument");
+    const void* src = nullptr;
+    bool convert = false;
+    switch (field) {
+    case NICR_FIELD_O: src = c->o[c->cur]; break;
+    case NICR_FIELD_CR: src = c->cr[c->cur]; break;
+    case NICR_FIELD_PHI: src = c->p[c->cur]; break;
+    case NICR_FIELD_GB: src = c->gb[mask_index(c)]; convert = c->te == 4; break;
+    case NICR_FIELD_ETA_S2:
+    case NICR_FIELD_ETA_PAIRS:
+        if (!c->has_eta()) return fail(NICR_E_INVALID, "nicr_download_field: no eta fields");
+        src = field == NICR_FIELD_ETA_S2 ? c->s2[mask_index(c)] : c->pairs[mask_index(c)];
+        convert = c->te == 4;
+        break;
+    case NICR_FIELD_MU_O:
+    case NICR_FIELD_MU_CR:
+    case NICR_FIELD_M_O:
+    case NICR_FIELD_M_CR:
+        if (!c->refresh_valid) return fail(NICR_E_STATE, "nicr_download_field: call nicr_refresh first");
+        src = c->out4[field - NICR_FIELD_MU_O];
+        break;
+    default:
+        if (field >= NICR_FIELD_ETA0 && field < NICR_FIELD_ETA0 + c->cfg.n_eta) {
+            src = eta_ptr(c, mask_index(c), field - NICR_FIELD_ETA0);
+            convert = c->te == 4;
+        } else {
+            return fail(NICR_E_INVALID, "nicr_download_field: unknown field");
+        }
+    }
+    if ((field == NICR_FIELD_GB || field == NICR_FIELD_ETA_S2 || field == NICR_FIELD_ETA_PAIRS) && c->mask_dirty) {
+        do_mask(c);
+        c->mask_dirty = false;
+    }
+    if (convert) {
+        std::vector<float> tmp((size_t)c->ncell);
+        HIPCHK(hipMemcpyAsync(tmp.data(), src, tmp.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < tmp.size(); ++i) dstp[i] = (double)tmp[i];
+    } else {
+        HIPCHK(hipMemcpyAsync(dstp, src, (size_t)c->ncell * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return NICR_OK;
+}
+
+int nicr_set_budget(void* h, const nicr_budget* b) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c || !b) return fail(NICR_E_INVALID, "null argument");
+    DevCtl hc;
+    int rc = sync_ctl(c, &hc);
+    if (rc) return rc;
+    const double* src[8] = {b->nucO, b->nucCr, b->ckO, b->ckCr, b->clipO, b->clipCr, b->bcO, b->bcCr};
+    for (int s = 0; s < 8; ++s) {
+        hc.budget[2 * s] = src[s][0];
+        hc.budget[2 * s + 1] = src[s][1];
+    }
+    hc.nuclei = b->nuclei;
+    hc.clipped = b->clipped;
+    HIPCHK(hipMemcpyAsync(c->ctl, &hc, sizeof(hc), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NICR_OK;
+}
+
+int nicr_get_budget(void* h, nicr_budget* b) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c || !b) return fail(NICR_E_INVALID, "null argument");
+    DevCtl hc;
+    int rc = sync_ctl(c, &hc);
+    if (rc) return rc;
+    double* dst[8] = {b->nucO, b->nucCr, b->ckO, b->ckCr, b->clipO, b->clipCr, b->bcO, b->bcCr};
+    for (int s = 0; s < 8; ++s) {
+        dst[s][0] = hc.budget[2 * s];
+        dst[s][1] = hc.budget[2 * s + 1];
+    }
+    b->nuclei = hc.nuclei;
+    b->clipped = hc.clipped;
+    return NICR_OK;
+}
+
+int nicr_refresh(void* h) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c) return fail(NICR_E_INVALID, "null context");
+    if (!c->out4[0]) {
+        for (int i = 0; i < 4; ++i) {
+            int rc = dalloc(c, (void**)&c->out4[i], (size_t)c->ncell * 8);
+            if (rc) return rc;
+        }
+    }
+    if (c->mask_dirty) {
+        do_mask(c);
+        c->mask_dirty = false;
+    }
+    do_refresh_out(c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->refresh_valid = true;
+    return NICR_OK;
+}
+
+int nicr_step(void* h, int nsteps, nicr_step_report* rep) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c || nsteps < 0) return fail(NICR_E_INVALID, "nicr_step: bad argument");
+    if (c->mask_dirty) {
+        do_mask(c);
+        c->mask_dirty = false;
+    }
+    hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(1), 0, c->stream, c->ctl);
+    for (int s = 0; s < nsteps; ++s) do_step(c);
+    HIPCHK(hipGetLastError());
+    DevCtl hc;
+    int rc = sync_ctl(c, &hc);
+    if (rc) return rc;
+    tcollect(c);
+    c->refresh_valid = false;
+    nicr_step_report r{};
+    r.steps_done = hc.steps_done;
+    r.status = NICR_STEP_OK;
+    if (hc.failed) {
+        const int nxt = hc.cur ^ 1;
+        const long long nyz = (long long)c->P.nyz;
+        if (hc.fail_kind == 1) {
+            const long long q = (long long)hc.pde_key;
+            double v[3];
+            HIPCHK(hipMemcpy(&v[0], c->o[nxt] + q, 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(&v[1], c->cr[nxt] + q, 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(&v[2], c->p[nxt] + q, 8, hipMemcpyDeviceToHost));
+            r.o = v[0];
+            r.cr = v[1];
+            r.phi = v[2];
+            r.status = (std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2])) ? NICR_STEP_OUT_OF_BOUNDS
+                                                                                            : NICR_STEP_NONFINITE;
+            r.i = (int)(q / nyz) + 1;
+            const long long yz = q % nyz;
+            r.j = (int)(yz / c->P.nz) + 1;
+            r.k = c->dim == 3 ? (int)(yz % c->P.nz) + 1 : 0;
+        } else {
+            const long long key = (long long)hc.eta_key;
+            const long long g = key / c->ncell, q = key % c->ncell;
+            r.status = NICR_STEP_ETA_RANGE;
+            r.grain = (int)g + 1;
+            if (c->te == 8) {
+                HIPCHK(hipMemcpy(&r.eta_value, (double*)c->eta[nxt] + key, 8, hipMemcpyDeviceToHost));
+            } else {
+                float f;
+                HIPCHK(hipMemcpy(&f, (float*)c->eta[nxt] + key, 4, hipMemcpyDeviceToHost));
+                r.eta_value = f;
+            }
+            r.i = (int)(q / nyz) + 1;
+            const long long yz = q % nyz;
+            r.j = (int)(yz / c->P.nz) + 1;
+            r.k = c->dim == 3 ? (int)(yz % c->P.nz) + 1 : 0;
+        }
+        // mpi2d keeps the concentration-BC reset of a rejected step (2D:552).
+        if (c->dim == 2 && c->P.enforce_layer) {
+            hipLaunchKernelGGL(k_apply_layer, dim3((c->P.nyz + 255) / 256), dim3(256), 0, c->stream, c->P,
+                               (double* const*)c->dev_obuf, (const DevCtl*)c->ctl);
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+    }
+    if (rep) *rep = r;
+    return NICR_OK;
+}
+
+int nicr_stamp(void* h, int x, int y, int z, int* changed) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c) return fail(NICR_E_INVALID, "null context");
+    if (x < 1 || x > c->P.nx || y < 1 || y > c->P.ny || (c->dim == 3 && (z < 1 || z > c->P.nz)))
+        return fail(NICR_E_INVALID, "nicr_stamp: centre outside the mesh");
+    int rc = ensure_nuc_scratch(c);
+    if (rc) return rc;
+    NucPar N{x, y, c->dim == 3 ? z : 1, c->cfg.radius_grid, c->cfg.exclusion_gap_grid, 4096};
+    if (c->dim == 2) {
+        if (c->te == 8) launch_stamp<2, double>(c, N);
+        else launch_stamp<2, float>(c, N);
+    } else {
+        if (c->te == 8) launch_stamp<3, double>(c, N);
+        else launch_stamp<3, float>(c, N);
+    }
+    HIPCHK(hipGetLastError());
+    int n = 0;
+    HIPCHK(hipMemcpyAsync(&n, c->nuc_misc + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (n && c->has_eta()) c->mask_dirty = true;
+    c->refresh_valid = false;
+    if (changed) *changed = n;
+    return NICR_OK;
+}
+
+int nicr_nucleate(void* h, int* inserted) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c) return fail(NICR_E_INVALID, "null context");
+    if (inserted) *inserted = 0;
+    if (c->cfg.nucleation != NICR_NUC_KSP) return NICR_OK;   // 2D:545
+    int rc = ensure_nuc_scratch(c);
+    if (rc) return rc;
+    NucPar N{0, 0, 0, c->cfg.radius_grid, c->cfg.exclusion_gap_grid, 4096};
+    if (c->dim == 2) {
+        if (c->te == 8) launch_nucleate<2, double>(c, N);
+        else launch_nucleate<2, float>(c, N);
+    } else {
+        if (c->te == 8) launch_nucleate<3, double>(c, N);
+        else launch_nucleate<3, float>(c, N);
+    }
+    HIPCHK(hipGetLastError());
+    int n = 0;
+    HIPCHK(hipMemcpyAsync(&n, c->nuc_misc + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (n && c->has_eta()) c->mask_dirty = true;
+    c->refresh_valid = false;
+    if (inserted) *inserted = n;
+    return NICR_OK;
+}
+
+int nicr_field_stats(void* h, int field, nicr_stats* out) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c || !out) return fail(NICR_E_INVALID, "null argument");
+    const double* src = nullptr;
+    int mode = 0;
+    switch (field) {
+    case NICR_FIELD_O: src = c->o[c->cur]; break;
+    case NICR_FIELD_CR: src = c->cr[c->cur]; break;
+    case NICR_FIELD_PHI: src = c->p[c->cur]; break;
+    default: return fail(NICR_E_INVALID, "nicr_field_stats: O, CR or PHI only");
+    }
+    hipLaunchKernelGGL((k_stats<double>), dim3(c->g_stat), dim3(256), 0, c->stream, src, (const double*)nullptr,
+                       c->ncell, mode, c->stat_part);
+    std::vector<double> part((size_t)c->g_stat * 4);
+    HIPCHK(hipMemcpyAsync(part.data(), c->stat_part, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    long double s = 0;
+    double mn = INFINITY, mx = -INFINITY;
+    for (int b = 0; b < c->g_stat; ++b) {
+        s += (long double)part[4 * b] + (long double)part[4 * b + 1];
+        mn = std::min(mn, part[4 * b + 2]);
+        mx = std::max(mx, part[4 * b + 3]);
+    }
+    out->sum[0] = (double)s;
+    out->sum[1] = (double)(s - (long double)out->sum[0]);
+    out->min = mn;
+    out->max = mx;
+    return NICR_OK;
+}
+
+int nicr_sum_h_phi(void* h, double out_sum[2]) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c || !out_sum) return fail(NICR_E_INVALID, "null argument");
+    hipLaunchKernelGGL((k_stats<double>), dim3(c->g_stat), dim3(256), 0, c->stream, (const double*)c->p[c->cur],
+                       (const double*)nullptr, c->ncell, 1, c->stat_part);
+    std::vector<double> part((size_t)c->g_stat * 4);
+    HIPCHK(hipMemcpyAsync(part.data(), c->stat_part, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    long double s = 0;
+    for (int b = 0; b < c->g_stat; ++b) s += (long double)part[4 * b] + (long double)part[4 * b + 1];
+    out_sum[0] = (double)s;
+    out_sum[1] = (double)(s - (long double)out_sum[0]);
+    return NICR_OK;
+}
+
+int nicr_set_timing(void* h, int enabled) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c) return fail(NICR_E_INVALID, "null context");
+    c->timing = enabled != 0;
+    return NICR_OK;
+}
+int nicr_reset_kernel_times(void* h) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c) return fail(NICR_E_INVALID, "null context");
+    for (auto& t : c->timers) {
+        t.total_ms = 0.;
+        t.count = 0;
+    }
+    return NICR_OK;
+}
+int nicr_kernel_times(void* h, char* buf, size_t len) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c || !buf || len == 0) return fail(NICR_E_INVALID, "null argument");
+    std::string s;
+    int n = 0;
+    for (auto& t : c->timers) {
+        if (!t.count) continue;
+        char line[128];
+        std::snprintf(line, sizeof line, "%s=%.6f:%lld;", t.name, t.total_ms / t.count, t.count);
+        s += line;
+        ++n;
+    }
+    std::snprintf(buf, len, "%s", s.c_str());
+    return n;
+}
+int nicr_sync(void* h) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c) return fail(NICR_E_INVALID, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NICR_OK;
+}
+
+}  // extern "C"

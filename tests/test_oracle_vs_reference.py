@@ -1,0 +1,149 @@
+"""Pins the CPU oracle to the reference itself.
+
+The reference's `class Model` is compiled from /root/reference into oracle/_ref
+(oracle/Makefile) and driven through the same C shape as the oracle. Starting
+from the reference's own constructed state (Voronoi mask, eta, centre seed),
+both take identical steps; every field must agree bit for bit and every
+long-double budget exactly. Skipped where the reference sources are absent.
+"""
+import numpy as np
+import pytest
+
+CASES = [
+    (2, dict(nx=31, ny=25, grain_diameter_grid=10., bc="concentration", bounds="legacy", dt=1e-12)),
+    (2, dict(nx=35, ny=29, grain_diameter_grid=16., bc="mu", bounds="legacy", grain_evolution=True,
+             nucleation="center", radius_grid=3.5, dt=1e-12)),
+    (2, dict(nx=33, ny=21, grains=False, bc="mu", ck="off", bounds="legacy", dt=2e-10)),
+    (2, dict(nx=61, ny=61, grain_diameter_grid=50., random_seed=2001, left_edge_sites=2, grain_evolution=True,
+             nucleation="center", radius_grid=5.5, bc="closed", ck="off", bounds="legacy", dt=1e-12)),
+    (3, dict(nx=23, ny=13, nz=11, grain_diameter_grid=12., bc="concentration", bounds="legacy", dt=1e-12)),
+    (3, dict(nx=23, ny=13, nz=11, grain_diameter_grid=12., bc="mu", bounds="legacy", grain_evolution=True,
+             nucleation="center", radius_grid=2.5, dt=1e-12)),
+    (3, dict(nx=23, ny=13, nz=11, grain_diameter_grid=12., bc="closed", ck="off", bounds="stop",
+             grain_evolution=True, eta_L=0.0, dt=1e-12)),
+    (3, dict(nx=21, ny=17, nz=15, grains=False, bc="concentration", bounds="legacy", oxygen_diffusion=False)),
+    (3, dict(nx=25, ny=15, nz=13, grain_diameter_grid=12., bc="mu", bounds="legacy", ck="legacy", dt=2e-10)),
+]
+
+
+def _fields(m):
+    names = ["o", "cr", "p", "gb", "muO", "muCr", "mo", "mc"] + list(range(m.n_eta))
+    return names
+
+
+def _pair(nicr, oracle, dim, kw):
+    if not oracle.ref_available(dim):
+        pytest.skip("reference sources not built into oracle/_ref")
+    cfg = nicr.make_config(dim, **kw)
+    ref = oracle.RefModel(cfg)
+    cfg.n_eta = ref.n_eta
+    ora = oracle.OracleModel(cfg)
+    oracle.copy_state(ref, ora)
+    return ref, ora
+
+
+def _assert_same(ref, ora, tag):
+    ref.refresh()
+    ora.refresh()
+    for f in _fields(ora):
+        a, b = ref.get(f), ora.get(f)
+        assert np.array_equal(a, b, equal_nan=True), f"{tag}: {f} max diff {np.nanmax(np.abs(a - b))}"
+    rb, ob = ref.budget_raw(), ora.budget_raw()
+    for k in ("nucO", "nucCr", "ckO", "ckCr", "clipO", "clipCr", "bcO", "bcCr"):
+        assert tuple(getattr(rb, k)) == tuple(getattr(ob, k)), f"{tag}: budget {k}"
+    assert (rb.nuclei, rb.clipped) == (ob.nuclei, ob.clipped)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_step_bitwise(nicr, oracle, case):
+    dim, kw = CASES[case]
+    ref, ora = _pair(nicr, oracle, dim, kw)
+    for n in range(12):
+        st, done, msg = ref.step(1)
+        r = ora.try_step(1)
+        assert st == r.status
+        if st != 0:
+            assert msg == nicr.step_error_message(r, dim)
+            break
+    _assert_same(ref, ora, f"case {case}")
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_rejection_message_and_rollback(nicr, oracle, dim):
+    kw = dict(nx=21, ny=9, bc="concentration", bounds="stop", grains=False)
+    if dim == 3:
+        kw.update(nz=7)
+    ref, ora = _pair(nicr, oracle, dim, kw)
+    for m in (ref, ora):
+        cr = m.get("cr")
+        if dim == 2:
+            cr[19, 4] = -0.1
+        else:
+            cr[19, 3, 2] = -0.1
+        m.set("cr", cr)
+    st, done, msg = ref.step(1)
+    r = ora.try_step(1)
+    assert st == r.status == 2
+    assert msg == nicr.step_error_message(r, dim)
+    _assert_same(ref, ora, "after rejection")
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_eta_rejection(nicr, oracle, dim):
+    kw = dict(nx=21, ny=9, grain_diameter_grid=4., grain_evolution=True, bc="closed", bounds="legacy",
+              ck="off", nucleation="off")
+    if dim == 3:
+        kw.update(nz=7)
+    ref, ora = _pair(nicr, oracle, dim, kw)
+    for m in (ref, ora):
+        e = m.get(0)
+        if dim == 2:
+            e[20, 4] = 2.0
+        else:
+            e[20, 3, 2] = 2.0
+        m.set(0, e)
+    st, done, msg = ref.step(1)
+    r = ora.try_step(1)
+    assert st == r.status == 3
+    assert msg == nicr.step_error_message(r, dim)
+    _assert_same(ref, ora, "after eta rejection")
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_ksp_scan_and_stamp(nicr, oracle, dim):
+    kw = dict(nx=61 if dim == 2 else 31, ny=31 if dim == 2 else 13, grain_diameter_grid=20. if dim == 2 else 12.,
+              bc="closed", ck="off", bounds="legacy", radius_grid=3. if dim == 2 else 2.5, dt=1e-12,
+              grain_evolution=True)
+    if dim == 3:
+        kw.update(nz=11)
+    ref, ora = _pair(nicr, oracle, dim, kw)
+    x = 15
+    args = (x, 1) if dim == 2 else (x, 1, 1)
+    assert ref.stamp(*args) == ora.stamp(*args)
+    _assert_same(ref, ora, "after stamp")
+    for m in (ref, ora):
+        m.set("o", np.full(m.shape, .001))
+    ref.nucleate()
+    ora.nucleate()
+    _assert_same(ref, ora, "after Ksp scan")
+    # the exclusion predicate itself, over every cell
+    for i in range(1, kw["nx"] + 1, 3):
+        for j in range(1, kw["ny"] + 1, 2):
+            k = 1 if dim == 2 else (j % kw["nz"]) + 1
+            assert ref.valid_nucleus(i, j, k) == ora.valid_nucleus(i, j, k)
+    for n in range(3):
+        ref.step(1)
+        ora.try_step(1)
+    _assert_same(ref, ora, "after post-nucleation steps")
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_sum_long_double(nicr, oracle, dim):
+    kw = dict(nx=31, ny=25, bc="concentration", bounds="legacy", grain_diameter_grid=10.)
+    if dim == 3:
+        kw.update(nz=11)
+    ref, ora = _pair(nicr, oracle, dim, kw)
+    ref.step(3)
+    ora.try_step(3)
+    for f in ("o", "cr", "p"):
+        assert ref.sum(f) == ora.sum(f)
