@@ -91,7 +91,7 @@ def cpu_baseline(nicr, n_eta, seed):
     for g in range(n_eta):
         ora.set(g, eta[g])
     del state, eta
-    nsteps = 2
+    nsteps = 12
     t0 = time.perf_counter()
     done = ora.run(nsteps)
     t1 = time.perf_counter()
@@ -111,18 +111,28 @@ def main():
     ap.add_argument("--seeds", type=int, default=64)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-only", action="store_true",
+                    help="run only the 1 GiB device copy reference (rocprof counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the X-slab sharding design is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
     nicr = _pkg()
+    if args.copy_probe_only:
+        cfg = nicr.make_config(3, nx=8, ny=8, nz=8)
+        m = nicr.Model(cfg)
+        ms = m.copy_probe(1 << 30, 20)
+        print(json.dumps({"copy_probe_bytes": 1 << 30, "ms": ms, "GBs_read_plus_write": 2 * (1 << 30) / ms / 1e6}))
+        return
     shape = (args.n, args.n, args.n)
     cells = shape[0] * shape[1] * shape[2]
     state = build_state(nicr, shape, args.seeds, args.eta, 20260909)
 
     m = make_model(nicr, shape, args.eta, "mixed", state)
     print(f"[bench] mixed model resident: {m.device_bytes() / 2**30:.2f} GiB", file=sys.stderr, flush=True)
+    copy_ms = m.copy_probe(1 << 30, 10)
+    copy_gbs = 2 * (1 << 30) / copy_ms / 1e6
     wall, kt = time_steps(m, args.steps, args.warmup, "mixed")
     m.close()
     ms_per_step = wall / args.steps * 1e3
@@ -164,6 +174,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_eta", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "fp64": fp64,
+        "copy_probe_GBs": round(copy_gbs, 1),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nicr, args.eta, 20260909)

@@ -170,6 +170,11 @@ int nicr_reset_kernel_times(void* ctx);
 int nicr_set_timing(void* ctx, int enabled);
 /* Block until the device has finished all work queued on the context. */
 int nicr_sync(void* ctx);
+/* Plain device-to-device copy of `bytes` bytes repeated `reps` times (kernel
+ * k_copy_probe); mean ms per copy. The bandwidth reference against which the
+ * step kernels' HBM byte counters and roofline fractions are calibrated. No
+ * counterpart in the reference. */
+int nicr_copy_probe(void* ctx, size_t bytes, int reps, double* ms);
 
 #ifdef __cplusplus
 }

@@ -127,6 +127,7 @@ def load_library(path: str = LIB_PATH):
     lib.nicr_set_timing.argtypes = [_VOIDP, ctypes.c_int]
     lib.nicr_sync.argtypes = [_VOIDP]
     lib.nicr_device_bytes.argtypes = [_VOIDP, ctypes.POINTER(ctypes.c_size_t)]
+    lib.nicr_copy_probe.argtypes = [_VOIDP, ctypes.c_size_t, ctypes.c_int, _DPTR]
     _LIB = lib
     return lib
 
@@ -339,6 +340,12 @@ class Model:
 
     def sync(self) -> None:
         self._h.check(self._h.lib.nicr_sync(self._h.h))
+
+    def copy_probe(self, nbytes: int, reps: int = 10) -> float:
+        """Mean ms of a plain device copy of nbytes (bandwidth reference)."""
+        ms = ctypes.c_double(0.0)
+        self._h.check(self._h.lib.nicr_copy_probe(self._h.h, int(nbytes), int(reps), ctypes.byref(ms)))
+        return ms.value
 
     def device_bytes(self) -> int:
         n = ctypes.c_size_t(0)

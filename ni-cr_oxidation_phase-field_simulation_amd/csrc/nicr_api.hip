@@ -69,6 +69,10 @@ struct Ctx {
     int cur = 0;                 // host mirror of ctl->cur
     bool mask_dirty = true;
     bool refresh_valid = false;
+    bool march_disabled = false;   // NICR_KERNEL_VARIANT=basic selects the unblocked kernels
+    int eta_ty = 8;                // Y tile height of the marching eta kernel (NICR_ETA_TY: 2, 4, 8)
+    int eta_nseg = 0;              // X segments per column (NICR_ETA_NSEG; 0: one wave of resident blocks)
+    int eta_ng = 16;               // max eta fields per marching pass (NICR_ETA_NG: 8, 16, 32)
     size_t bytes = 0;
     std::vector<void*> allocs;
     bool timing = false;
@@ -144,6 +148,62 @@ void launch_mask(Ctx* c) {
     hipLaunchKernelGGL((k_mask<DIM, PREC, TE, TM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P,
                        bufs<TE, TM>(c), c->eta_evolves() ? -1 : 0);
 }
+// X-marching eta kernel for 3D meshes with up to 32 eta fields.
+template <int PREC, int NG, int TY, class TE, class TM>
+void launch_eta_march_ng(Ctx* c, const Bufs<TE, TM>& B) {
+    const Par& P = c->P;
+    const int nTz = (P.nz + 63) / 64, nTy = (P.ny + TY - 1) / TY;
+    const int tiles = nTz * nTy;
+    const size_t lds = (size_t)NG * (TY + 2) * 66 * sizeof(TE);
+    static int resident = 0;   // blocks of this instantiation that fit on the chip at once
+    if (!resident) {
+        (void)hipFuncSetAttribute((const void*)k_eta_march<PREC, NG, TY, TE, TM>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int per_cu = 0, ncu = 0, dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eta_march<PREC, NG, TY, TE, TM>, 64 * TY,
+                                                           lds);
+        resident = std::max(1, per_cu) * std::max(1, ncu);
+    }
+    // one wave of resident blocks: X segments long enough to amortise the
+    // extra plane each segment loads at its start
+    int nseg = std::max(1, std::min((resident + tiles - 1) / tiles, P.nx));
+    if (c->eta_nseg > 0) nseg = std::min(c->eta_nseg, P.nx);
+    const int XS = (P.nx + nseg - 1) / nseg;
+    nseg = (P.nx + XS - 1) / XS;
+    for (int g0 = 0; g0 < P.n_eta; g0 += NG)
+        hipLaunchKernelGGL((k_eta_march<PREC, NG, TY, TE, TM>), dim3(tiles * nseg), dim3(64 * TY), lds, c->stream, P,
+                           B, XS, g0);
+}
+template <int PREC, int TY, class TE, class TM>
+void launch_eta_march_ty(Ctx* c, const Bufs<TE, TM>& B) {
+    const int n = c->P.n_eta;
+    // fields per pass: the smallest instantiation that holds them, capped by
+    // eta_ng (passes keep the per-XCD plane working set inside L2)
+    const int cap = c->eta_ng;
+    if (n <= 4) launch_eta_march_ng<PREC, 4, TY>(c, B);
+    else if (n <= 8 || cap <= 8) launch_eta_march_ng<PREC, 8, TY>(c, B);
+    else if (n <= 16 || cap <= 16) launch_eta_march_ng<PREC, 16, TY>(c, B);
+    else launch_eta_march_ng<PREC, 32, TY>(c, B);
+}
+template <int PREC, class TE, class TM>
+bool launch_eta_march(Ctx* c, const Bufs<TE, TM>& B) {
+    const int n = c->P.n_eta;
+    if (c->march_disabled || c->dim != 3 || n > 32 || c->P.nz < 32) return false;
+    if constexpr (PREC == PREC_FP64) {
+        // binary64 register planes of more than 8 fields limit occupancy to one
+        // wave per SIMD; the unblocked kernel is faster there
+        if (n > 8) return false;
+        launch_eta_march_ty<PREC, 2>(c, B);
+    } else {
+        if (c->eta_ty == 2) launch_eta_march_ty<PREC, 2>(c, B);
+        else if (c->eta_ty == 8) launch_eta_march_ty<PREC, 8>(c, B);
+        else launch_eta_march_ty<PREC, 4>(c, B);
+    }
+    return true;
+}
+
 template <int DIM, int PREC, class TE, class TM>
 void launch_step(Ctx* c) {
     auto B = bufs<TE, TM>(c);
@@ -155,7 +215,10 @@ void launch_step(Ctx* c) {
     tmark_end(c);
     if (c->eta_evolves()) {
         tmark_begin(c, 2);
-        hipLaunchKernelGGL((k_eta<DIM, PREC, TE, TM>), dim3(c->g_eta), dim3(TILE), 0, c->stream, c->P, B);
+        bool done = false;
+        if constexpr (DIM == 3) done = launch_eta_march<PREC, TE, TM>(c, B);
+        if (!done)
+            hipLaunchKernelGGL((k_eta<DIM, PREC, TE, TM>), dim3(c->g_eta), dim3(TILE), 0, c->stream, c->P, B);
         tmark_end(c);
     }
     tmark_begin(c, 3);
@@ -227,6 +290,20 @@ void do_refresh_out(Ctx* c) {
         if (c->dim == 2) launch_refresh_out<2, float, float>(c);
         else launch_refresh_out<3, float, float>(c);
     }
+}
+
+__global__ __launch_bounds__(256) void k_copy_probe(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+    // four independent 16-byte loads in flight per lane
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const float4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
+        b[i] = v0;
+        b[i + stride] = v1;
+        b[i + 2 * stride] = v2;
+        b[i + 3 * stride] = v3;
+    }
+    for (; i < n; i += stride) b[i] = a[i];
 }
 
 __global__ void k_reset_ctl(DevCtl* ctl) {
@@ -332,6 +409,10 @@ int nicr_create(const nicr_config* cfg, void** out) {
 
     Ctx* c = new Ctx;
     c->cfg = *cfg;
+    if (const char* kv = std::getenv("NICR_KERNEL_VARIANT")) c->march_disabled = std::strcmp(kv, "basic") == 0;
+    if (const char* ty = std::getenv("NICR_ETA_TY")) c->eta_ty = std::atoi(ty);
+    if (const char* ns = std::getenv("NICR_ETA_NSEG")) c->eta_nseg = std::atoi(ns);
+    if (const char* ng = std::getenv("NICR_ETA_NG")) c->eta_ng = std::atoi(ng);
     if (c->cfg.dim == 2) c->cfg.nz = 1;
     if (!(c->cfg.grains && c->cfg.grain_evolution)) c->cfg.n_eta = 0;
     c->dim = c->cfg.dim;
@@ -800,6 +881,36 @@ int nicr_kernel_times(void* h, char* buf, size_t len) {
     std::snprintf(buf, len, "%s", s.c_str());
     return n;
 }
+// Plain device copy of `bytes` bytes, `reps` times, on the context stream: the
+// bandwidth reference that rocprof byte counters and roofline fractions are
+// calibrated against. Returns the mean duration per copy in *ms.
+int nicr_copy_probe(void* h, size_t bytes, int reps, double* ms) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c || !ms || reps < 1 || bytes < 1024) return fail(NICR_E_INVALID, "nicr_copy_probe: bad argument");
+    const size_t n4 = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, n4 * 16));
+    HIPCHK(hipMalloc(&b, n4 * 16));
+    HIPCHK(hipMemsetAsync(a, 0, n4 * 16, c->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, c->stream, (const float4*)a, (float4*)b, n4);
+    HIPCHK(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, c->stream, (const float4*)a, (float4*)b, n4);
+    HIPCHK(hipEventRecord(e1, c->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    return NICR_OK;
+}
+
 int nicr_sync(void* h) {
     Ctx* c = static_cast<Ctx*>(h);
     if (!c) return fail(NICR_E_INVALID, "null context");

@@ -476,6 +476,209 @@ __global__ __launch_bounds__(TILE) void k_eta(Par P, Bufs<TE, TM> B) {
 }
 
 // ---------------------------------------------------------------------------
+// k_eta_march: the same eta trial, restructured so each eta value crosses HBM
+// once. A block owns a 64 (z) x TY (y) column tile of all NG eta fields and
+// marches along X over a segment of XS planes. Planes x-1, x, x+1 of every
+// field sit in registers (X neighbours), plane x with a one-cell Y/Z halo sits
+// in LDS (Y/Z neighbours). Arithmetic is identical to k_eta (same operation
+// order in binary64), so both kernels give bitwise equal results.
+// ---------------------------------------------------------------------------
+// Fields are processed in passes of NG (g0 = first field of this pass) so that
+// the plane working set of one XCD fits its L2; the running sums S and P of the
+// next-step mask are carried between passes in the output mask buffers, which
+// keeps the accumulation order, and so the result, identical to one pass.
+template <int PREC, int NG, int TY, class TE, class TM>
+__global__ __launch_bounds__(64 * TY) void k_eta_march(Par P, Bufs<TE, TM> B, int XS, int g0) {
+    using T = TE;
+    constexpr int W = 66;                 // LDS row: 64 interior + 2 halo
+    constexpr int PL = (TY + 2) * W;      // one field's plane tile
+    extern __shared__ unsigned char lds_raw[];
+    T* lds = reinterpret_cast<T*>(lds_raw);
+    DevCtl* ctl = B.ctl;
+    if (ctl->failed) return;
+    const int cur_i = ctl->cur, nxt_i = cur_i ^ 1;
+    const int tz = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int nTz = (P.nz + 63) >> 6, nTy = (P.ny + TY - 1) / TY, nSeg = (P.nx + XS - 1) / XS;
+    const int b = xcd_remap(blockIdx.x, nTz * nTy * nSeg);
+    const int seg = b / (nTz * nTy);
+    const int rem = b - seg * (nTz * nTy);
+    const int tyT = rem / nTz, tzT = rem - tyT * nTz;
+    const int z0 = tzT * 64, y0 = tyT * TY;
+    const int z = z0 + tz, y = y0 + ty;
+    const bool active = z < P.nz && y < P.ny;
+    const int x0 = seg * XS, x1 = min(P.nx, x0 + XS);
+    const long long plane = P.nyz;
+    // Lanes past the mesh edge of a partial tile hold the periodically wrapped
+    // cell, so the LDS tile row/column after the last active one is exactly
+    // the Y/Z periodic neighbour; they compute and store nothing.
+    const long long rc = (long long)(y % P.ny) * P.nz + (z % P.nz);
+    const TE* ein = B.eta[cur_i] + (long long)g0 * P.ncell;
+    TE* eout = B.eta[nxt_i] + (long long)g0 * P.ncell;
+    const int ng = min(NG, P.n_eta - g0);
+    const bool first_pass = g0 == 0, last_pass = g0 + NG >= P.n_eta;
+    const long long key0 = (long long)g0 * P.ncell;
+
+    T prev[NG], cv[NG], nx_[NG];
+    {
+        const int xp = x0 == 0 ? 0 : x0 - 1;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            if (g < ng) {
+                prev[g] = ein[(long long)g * P.ncell + xp * plane + rc];
+                cv[g] = ein[(long long)g * P.ncell + (long long)x0 * plane + rc];
+            } else {
+                prev[g] = T(0);
+                cv[g] = T(0);
+            }
+        }
+    }
+    // halo cells of the tile: row y0-1, row y0+TY (64 each), columns z0-1, z0+64 (TY each)
+    const int nhalo = 128 + 2 * TY;
+    unsigned long long bad = NO_KEY;
+    for (int x = x0; x < x1; ++x) {
+        const long long xo = (long long)x * plane;
+        const long long xn = (long long)(x == P.nx - 1 ? x : x + 1) * plane;
+        // issue the next plane's loads early (register prefetch)
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            nx_[g] = g < ng ? ein[(long long)g * P.ncell + xn + rc] : T(0);
+        // interior of plane x into LDS
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            if (g < ng) lds[g * PL + (ty + 1) * W + tz + 1] = cv[g];
+        // halo of plane x: work items (halo cell, chunk of 8 fields); each item
+        // issues 8 independent loads so that many are in flight per lane
+        constexpr int GC = NG < 8 ? NG : 8;
+        const int nchunk_g = (ng + GC - 1) / GC;
+        for (int qq = threadIdx.x; qq < nhalo * nchunk_g; qq += 64 * TY) {
+            const int gk = qq / nhalo;
+            const int hh = qq - gk * nhalo;
+            int ly, lz, gy, gz;
+            if (hh < 128) {
+                const int side = hh >> 6;   // 0: row y0-1, 1: row y0+TY
+                lz = (hh & 63) + 1;
+                ly = side ? TY + 1 : 0;
+                gy = side ? y0 + TY : y0 - 1;
+                gz = z0 + (hh & 63);
+            } else {
+                const int q = hh - 128;
+                const int side = q / TY;    // 0: column z0-1, 1: column z0+64
+                ly = q - side * TY + 1;
+                lz = side ? 65 : 0;
+                gy = y0 + ly - 1;
+                gz = side ? z0 + 64 : z0 - 1;
+            }
+            // clip to the mesh: a tile row/column beyond the mesh edge is never read
+            // by an active lane except through the periodic wrap below
+            if (gy >= P.ny && gy != y0 + TY) continue;
+            if (gz >= P.nz && gz != z0 + 64) continue;
+            gy = ((gy % P.ny) + P.ny) % P.ny;
+            gz = ((gz % P.nz) + P.nz) % P.nz;
+            const long long hc = xo + (long long)gy * P.nz + gz;
+            T hv[GC];
+#pragma unroll
+            for (int u = 0; u < GC; ++u) {
+                const int g = gk * GC + u;
+                hv[u] = g < ng ? ein[(long long)g * P.ncell + hc] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < GC; ++u) {
+                const int g = gk * GC + u;
+                if (g < ng) lds[g * PL + ly * W + lz] = hv[u];
+            }
+        }
+        __syncthreads();
+        if (active) {
+            const int lym = ty;
+            const int lzp = tz + 2, lzm = tz;
+            const long long c = xo + rc;
+            const double pp = B.p[cur_i][c];
+            if constexpr (PREC == PREC_FP64) {
+                const double S = (double)B.s2[cur_i][c];
+                double S2 = first_pass ? 0. : (double)B.s2[nxt_i][c];
+                double pr = first_pass ? 0. : (double)B.pairs[nxt_i][c];
+                const double dtl = P.dt * P.etaL;
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    if (g >= ng) break;
+                    const T* L = lds + g * PL;
+                    const double ec = (double)cv[g];
+                    const double eyp = (double)L[(ty + 2) * W + tz + 1];
+                    const double eym = (double)L[lym * W + tz + 1];
+                    const double ezp = (double)L[(ty + 1) * W + lzp];
+                    const double ezm = (double)L[(ty + 1) * W + lzm];
+                    const double xy = ((double)nx_[g] - 2. * ec + (double)prev[g]) / (P.dx * P.dx)
+                                    + (eyp - 2. * ec + eym) / (P.dy * P.dy);
+                    const double lp = xy + (ezp - 2. * ec + ezm) / (P.dz * P.dz);
+                    const double dF = ec * (P.etaW * (S - (1. - h64(pp)) + 2. * (S - ec * ec)) + P.etaA * h64(pp));
+                    const double v = ec - dtl * (dF - P.etaK * lp);
+                    if (!isfinite(v) || v < -1.e-12 || v > 1. + 1.e-12) {
+                        const unsigned long long key = (unsigned long long)(key0 + (long long)g * P.ncell + c);
+                        bad = bad < key ? bad : key;
+                    }
+                    eout[(long long)g * P.ncell + c] = (TE)v;
+                    const double q = v * v;
+                    pr += S2 * q;
+                    S2 += q;
+                }
+                B.s2[nxt_i][c] = (TE)S2;
+                B.pairs[nxt_i][c] = (TE)pr;
+                if (last_pass) {
+                    const double s16 = 16. * pr;
+                    B.gb[nxt_i][c] = (TE)(P.eta_scale * (s16 < 1. ? s16 : 1.));
+                }
+            } else {
+#pragma clang fp contract(fast)
+                const float S = (float)B.s2[cur_i][c];
+                const float hphi = h32((float)pp);
+                const float base = (float)P.etaW * (S - (1.f - hphi)) + 2.f * (float)P.etaW * S;
+                const float A = (float)P.etaA * hphi;
+                const float dtl = (float)(P.dt * P.etaL);
+                const float K = (float)P.etaK;
+                const float W2 = 2.f * (float)P.etaW;
+                float S2 = first_pass ? 0.f : (float)B.s2[nxt_i][c];
+                float pr = first_pass ? 0.f : (float)B.pairs[nxt_i][c];
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    if (g >= ng) break;
+                    const T* L = lds + g * PL;
+                    const float ec = (float)cv[g];
+                    const float eyp = (float)L[(ty + 2) * W + tz + 1];
+                    const float eym = (float)L[lym * W + tz + 1];
+                    const float ezp = (float)L[(ty + 1) * W + lzp];
+                    const float ezm = (float)L[(ty + 1) * W + lzm];
+                    const float lp = ((float)nx_[g] - 2.f * ec + (float)prev[g]) * P.fdx2i
+                                   + (eyp - 2.f * ec + eym) * P.fdy2i + (ezp - 2.f * ec + ezm) * P.fdz2i;
+                    const float dF = ec * (base - W2 * ec * ec + A);
+                    const float v = ec - dtl * (dF - K * lp);
+                    if (!isfinite(v) || v < -1.e-12f || v > 1.f + 1.e-12f) {
+                        const unsigned long long key = (unsigned long long)(key0 + (long long)g * P.ncell + c);
+                        bad = bad < key ? bad : key;
+                    }
+                    eout[(long long)g * P.ncell + c] = (TE)v;
+                    const float q = v * v;
+                    pr += S2 * q;
+                    S2 += q;
+                }
+                B.s2[nxt_i][c] = (TE)S2;
+                B.pairs[nxt_i][c] = (TE)pr;
+                if (last_pass) {
+                    const float s16 = 16.f * pr;
+                    B.gb[nxt_i][c] = (TE)((float)P.eta_scale * (s16 < 1.f ? s16 : 1.f));
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            prev[g] = cv[g];
+            cv[g] = nx_[g];
+        }
+    }
+    if (bad != NO_KEY) atomicMin(&ctl->eta_key, bad);
+}
+
+// ---------------------------------------------------------------------------
 // k_commit: all-or-nothing acceptance (collective_error + swap, 2D:594-596;
 // budget restore on rejection, SL:812-820). One block.
 // ---------------------------------------------------------------------------
