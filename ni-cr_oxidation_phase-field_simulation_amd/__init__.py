@@ -283,6 +283,9 @@ class Model:
         self._h.check(self._h._fn("get_budget")(self._h.h, ctypes.byref(bb)))
         return bb
 
+    def set_budget_raw(self, bb: NicrBudget) -> None:
+        self._h.check(self._h._fn("set_budget")(self._h.h, ctypes.byref(bb)))
+
     # ---- time step ----
     def refresh(self) -> None:
         self._h.check(self._h._fn("refresh")(self._h.h))

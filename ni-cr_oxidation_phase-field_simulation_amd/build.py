@@ -41,17 +41,18 @@ def build(quiet: bool = True) -> None:
     tmp = lib + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "nicr_api.hip", "-o", tmp], quiet)
     os.replace(tmp, lib)
-    host = os.path.join(CSRC, "host")
-    if os.path.isdir(host) and os.path.exists(os.path.join(host, "nicr_host.cpp")):
-        hlib = os.path.join(PKG, "libnicr_host.so")
-        _run(["g++", *HOST_FLAGS, "-shared", "host/nicr_host.cpp", "-o", hlib + ".tmp", "-lz"], quiet)
-        os.replace(hlib + ".tmp", hlib)
-        if os.path.exists(os.path.join(host, "nicr_cli.cpp")):
-            for dim in (2, 3):
-                exe = os.path.join(PKG, f"nicr{dim}d")
-                _run(["g++", *HOST_FLAGS, f"-DNICR_CLI_DIM={dim}", "host/nicr_cli.cpp", "-o", exe,
-                      f"-L{PKG}", "-lnicr_host", "-lnicr", f"-Wl,-rpath,$ORIGIN",
-                      "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-lz"], quiet)
+    # host side of the drop-in programs (g++): initial state, writers, CLI
+    hlib = os.path.join(PKG, "libnicr_host.so")
+    _run(["g++", *HOST_FLAGS, "-shared", "host/nicr_host.cpp", "-o", hlib + ".tmp",
+          f"-L{PKG}", "-lnicr", "-Wl,-rpath,$ORIGIN", "-lz"], quiet)
+    os.replace(hlib + ".tmp", hlib)
+    for dim in (2, 3):
+        exe = os.path.join(PKG, f"nicr{dim}d")
+        _run(["g++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", f"-DNICR_CLI_DIM={dim}",
+              "host/nicr_cli.cpp", "-o", exe + ".tmp",
+              f"-L{PKG}", "-lnicr_host", "-lnicr", "-Wl,-rpath,$ORIGIN",
+              "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib", "-lz"], quiet)
+        os.replace(exe + ".tmp", exe)
 
 
 if __name__ == "__main__":

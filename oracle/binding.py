@@ -105,6 +105,13 @@ class OracleModel:
     def try_step(self, nsteps: int = 1):
         return self._m.try_step(nsteps)
 
+    @property
+    def b(self) -> dict:
+        return self._m.b
+
+    def set_budget_raw(self, bb) -> None:
+        self._m.set_budget_raw(bb)
+
     def sum(self, name: str) -> float:
         nicr = product()
         out = (ctypes.c_double * 2)()

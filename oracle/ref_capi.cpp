@@ -157,6 +157,26 @@ int ref_sum(void* h,int id,double out[2]){
     if(!f){g_ref_error="unknown field";return NICR_E_INVALID;}
     split(Model::sum(*f),out);return NICR_OK;
 }
+// The reference's own writers, for byte comparisons with the drop-in writers.
+// what: 0 metadata into directory `path`, 1 diagnostics row, 2 grain rows
+// ("diag|history"), 3 legacy VTK, 4 VTI (3D only). Rows are appended.
+int ref_write(void* h,int what,const char* path,int step,const char* fmt){
+    auto& m=*static_cast<Handle*>(h)->m;
+    try{
+        if(what==0){m.c.out=path;m.metadata();}
+        else if(what==1){std::ofstream f(path,std::ios::app);m.log(f,step);}
+        else if(what==2){
+            const std::string s(path);const auto bar=s.find('|');
+            std::ofstream f(s.substr(0,bar),std::ios::app),g(s.substr(bar+1),std::ios::app);
+            m.grain_log(f,g,step);
+        }else if(what==3){m.vtk(path);}
+#if NICR_REF_DIM==3
+        else if(what==4){if(fmt&&*fmt)m.c.output_precision=fmt;m.vti(path);}
+#endif
+        else{g_ref_error="unknown writer";return NICR_E_INVALID;}
+    }catch(const std::exception& e){g_ref_error=e.what();return NICR_E_INVALID;}
+    (void)fmt;return NICR_OK;
+}
 // Timing helper for the CPU baseline: nsteps plain Model::step calls.
 int ref_run(void* h,int nsteps){
     auto& m=*static_cast<Handle*>(h)->m;
