@@ -66,6 +66,9 @@ struct Ctx {
     int* nuc_misc = nullptr;     // [0] candidate count, [1] changed/cells out
     int* nuc_centres = nullptr;
     int g_trial = 0, g_eta = 0, g_stat = 0;
+    int partial_slots = 0;         // budget partial slots allocated (one per block of the trial kernel)
+    int pde_ty = 4;                // Y tile height of the marching PDE kernel (NICR_PDE_TY: 2, 4)
+    int pde_nseg = 0;              // X segments of the marching PDE kernel (NICR_PDE_NSEG; 0: auto)
     int cur = 0;                 // host mirror of ctl->cur
     bool mask_dirty = true;
     bool refresh_valid = false;
@@ -204,15 +207,60 @@ bool launch_eta_march(Ctx* c, const Bufs<TE, TM>& B) {
     return true;
 }
 
+// Fused X-marching mu + trial kernel for 3D meshes. Returns the number of budget
+// partial slots it wrote (one per block).
+template <int PREC, int TY, class TE, class TM>
+int launch_pde_march_ty(Ctx* c, const Bufs<TE, TM>& B) {
+    const Par& P = c->P;
+    const int nTz = (P.nz + 63) / 64, nTy = (P.ny + TY - 1) / TY;
+    const int tiles = nTz * nTy;
+    const size_t lds = PdeLds<PREC, TY>::bytes();
+    static int resident = 0;
+    if (!resident) {
+        (void)hipFuncSetAttribute((const void*)k_pde_march<PREC, TY, TE, TM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        int per_cu = 0, ncu = 0, dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pde_march<PREC, TY, TE, TM>, 64 * TY, lds);
+        resident = std::max(1, per_cu) * std::max(1, ncu);
+    }
+    int nseg = std::max(1, std::min((resident + tiles - 1) / tiles, P.nx));
+    if (c->pde_nseg > 0) nseg = std::min(c->pde_nseg, P.nx);
+    while ((long long)tiles * nseg > c->partial_slots && nseg > 1) --nseg;
+    const int XS = (P.nx + nseg - 1) / nseg;
+    nseg = (P.nx + XS - 1) / XS;
+    if ((long long)tiles * nseg > c->partial_slots) return -1;
+    hipLaunchKernelGGL((k_pde_march<PREC, TY, TE, TM>), dim3(tiles * nseg), dim3(64 * TY), lds, c->stream, P, B, XS);
+    return tiles * nseg;
+}
+bool pde_march_applies(const Ctx* c) { return !c->march_disabled && c->dim == 3 && c->P.nz >= 32; }
+template <int PREC, class TE, class TM>
+int launch_pde_march(Ctx* c, const Bufs<TE, TM>& B) {
+    if (c->pde_ty == 2) return launch_pde_march_ty<PREC, 2>(c, B);
+    return launch_pde_march_ty<PREC, 4>(c, B);
+}
+
 template <int DIM, int PREC, class TE, class TM>
 void launch_step(Ctx* c) {
     auto B = bufs<TE, TM>(c);
-    tmark_begin(c, 0);
-    hipLaunchKernelGGL((k_mu<DIM, PREC, TE, TM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P, B);
-    tmark_end(c);
-    tmark_begin(c, 1);
-    hipLaunchKernelGGL((k_trial<DIM, PREC, TE, TM>), dim3(c->g_trial), dim3(TILE), 0, c->stream, c->P, B);
-    tmark_end(c);
+    int nparts = -1;
+    if constexpr (DIM == 3) {
+        if (pde_march_applies(c)) {
+            tmark_begin(c, 1);
+            nparts = launch_pde_march<PREC, TE, TM>(c, B);
+            tmark_end(c);
+        }
+    }
+    if (nparts < 0) {
+        tmark_begin(c, 0);
+        hipLaunchKernelGGL((k_mu<DIM, PREC, TE, TM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P, B);
+        tmark_end(c);
+        tmark_begin(c, 1);
+        hipLaunchKernelGGL((k_trial<DIM, PREC, TE, TM>), dim3(c->g_trial), dim3(TILE), 0, c->stream, c->P, B);
+        tmark_end(c);
+        nparts = c->g_trial;
+    }
     if (c->eta_evolves()) {
         tmark_begin(c, 2);
         bool done = false;
@@ -222,7 +270,7 @@ void launch_step(Ctx* c) {
         tmark_end(c);
     }
     tmark_begin(c, 3);
-    hipLaunchKernelGGL(k_commit, dim3(1), dim3(1024), 0, c->stream, bufs<double, double>(c), c->g_trial);
+    hipLaunchKernelGGL(k_commit, dim3(1), dim3(1024), 0, c->stream, bufs<double, double>(c), nparts);
     tmark_end(c);
 }
 template <int DIM, class TE, class TM>
@@ -413,6 +461,8 @@ int nicr_create(const nicr_config* cfg, void** out) {
     if (const char* ty = std::getenv("NICR_ETA_TY")) c->eta_ty = std::atoi(ty);
     if (const char* ns = std::getenv("NICR_ETA_NSEG")) c->eta_nseg = std::atoi(ns);
     if (const char* ng = std::getenv("NICR_ETA_NG")) c->eta_ng = std::atoi(ng);
+    if (const char* t = std::getenv("NICR_PDE_TY")) c->pde_ty = std::atoi(t);
+    if (const char* t = std::getenv("NICR_PDE_NSEG")) c->pde_nseg = std::atoi(t);
     if (c->cfg.dim == 2) c->cfg.nz = 1;
     if (!(c->cfg.grains && c->cfg.grain_evolution)) c->cfg.n_eta = 0;
     c->dim = c->cfg.dim;
@@ -497,7 +547,8 @@ int nicr_create(const nicr_config* cfg, void** out) {
     c->g_eta = std::min(P.ntiles, 2048);
     if (c->g_eta > 8) c->g_eta &= ~7;
     c->g_stat = 512;
-    if (rc == NICR_OK) rc = dalloc(c, (void**)&c->partials, (size_t)c->g_trial * NPART * 8);
+    c->partial_slots = std::max(c->g_trial, 8192);
+    if (rc == NICR_OK) rc = dalloc(c, (void**)&c->partials, (size_t)c->partial_slots * NPART * 8);
     if (rc == NICR_OK) rc = dalloc(c, (void**)&c->ctl, sizeof(DevCtl));
     if (rc == NICR_OK) rc = dalloc(c, (void**)&c->stat_part, (size_t)c->g_stat * 4 * 8);
     if (rc == NICR_OK) rc = dalloc(c, (void**)&c->dev_obuf, 6 * sizeof(double*));

@@ -70,35 +70,76 @@ __device__ __forceinline__ double lap64(const double* a, const Nbr& n, double ac
     return xy + (a[n.zp] - 2. * ac + a[n.zm]) / (P.dz * P.dz);
 }
 
+// Seven-point stencil values: centre, x-, x+, y-, y+, z-, z+ (2D: z entries
+// equal the centre and are not used). o carries the inlet substitution.
+enum { SC = 0, SXM = 1, SXP = 2, SYM = 3, SYP = 4, SZM = 5, SZP = 6 };
+
+// mu_O, mu_Cr at the centre from o, cr at 7 points and phi at the centre
+// (refresh hot loop 1, 2D:418-426, SL:604-612), binary64 in reference order.
 template <int DIM>
-__device__ __forceinline__ void mu64_at(const Par& P, const double* o, const double* cr, const double* p,
-                                         const Nbr& n, bool enforce, double& muO, double& muCr) {
-    const bool sub = enforce;
-    const double oc = (sub && n.x == 0) ? Phys::layerO : o[n.c];
-    const double oxm = (sub && n.x <= 1) ? Phys::layerO : o[n.xm];
-    const double oxp = (sub && n.x == 0 && P.nx == 1) ? Phys::layerO : o[n.xp];
-    const double cc = cr[n.c];
-    double hh = h64(p[n.c]);
-    // y/z neighbours share plane x; they are substituted iff the centre is.
-    double lo;
-    {
-        const double oyp = (sub && n.x == 0) ? Phys::layerO : o[n.yp];
-        const double oym = (sub && n.x == 0) ? Phys::layerO : o[n.ym];
-        double xy = (oxp - 2. * oc + oxm) / (P.dx * P.dx) + (oyp - 2. * oc + oym) / (P.dy * P.dy);
-        if (DIM == 3) {
-            const double ozp = (sub && n.x == 0) ? Phys::layerO : o[n.zp];
-            const double ozm = (sub && n.x == 0) ? Phys::layerO : o[n.zm];
-            xy = xy + (ozp - 2. * oc + ozm) / (P.dz * P.dz);
-        }
-        lo = xy;
-    }
+__device__ __forceinline__ void mu64_vals(const Par& P, const double* o7, const double* c7, double pc, double& muO,
+                                          double& muCr) {
+    const double oc = o7[SC], cc = c7[SC];
+    double hh = h64(pc);
+    double lo = (o7[SXP] - 2. * oc + o7[SXM]) / (P.dx * P.dx) + (o7[SYP] - 2. * oc + o7[SYM]) / (P.dy * P.dy);
+    if (DIM == 3) lo = lo + (o7[SZP] - 2. * oc + o7[SZM]) / (P.dz * P.dz);
+    double lc = (c7[SXP] - 2. * cc + c7[SXM]) / (P.dx * P.dx) + (c7[SYP] - 2. * cc + c7[SYM]) / (P.dy * P.dy);
+    if (DIM == 3) lc = lc + (c7[SZP] - 2. * cc + c7[SZM]) / (P.dz * P.dz);
     muO = (1. - hh) * mu_metal_o64(oc, cc)
         + hh * (24000000. * oc - 13800000. * cc - 8880000.) / Phys::Vm
         - 2. * Phys::ko * lo;
-    const double lc = lap64<DIM>(cr, n, cc, P, cr[n.xm], cr[n.xp]);
     muCr = (1. - hh) * (1166528.60789006 * cc + 3200000. * oc - 97045.64) / Phys::Vm
          + hh * (14000000. * cc - 13800000. * oc + 2680000.) / Phys::Vm
          - 2. * Phys::kc * lc;
+}
+
+// binary32 form: differences of the binary64 state are formed in binary64, the
+// rest in binary32; mu is stored as mu_O + 298432 and mu_Cr + 97045.64 (the
+// metal constants cancel in every face difference).
+template <int DIM>
+__device__ __forceinline__ void mu32_vals(const Par& P, const double* o7, const double* c7, double pc, float& muO,
+                                          float& muCr) {
+#pragma clang fp contract(fast)
+    const double oc = o7[SC], cc = c7[SC];
+    float lapo = (float)(o7[SXP] - 2. * oc + o7[SXM]) * P.fdx2i + (float)(o7[SYP] - 2. * oc + o7[SYM]) * P.fdy2i;
+    float lapc = (float)(c7[SXP] - 2. * cc + c7[SXM]) * P.fdx2i + (float)(c7[SYP] - 2. * cc + c7[SYM]) * P.fdy2i;
+    if (DIM == 3) {
+        lapo += (float)(o7[SZP] - 2. * oc + o7[SZM]) * P.fdz2i;
+        lapc += (float)(c7[SZP] - 2. * cc + c7[SZM]) * P.fdz2i;
+    }
+    const float hh = h32((float)pc);
+    const float fo = (float)oc, fc = (float)cc;
+    muO = (1.f - hh) * (12000000.f * fo + 3200000.f * fc) + hh * (24000000.f * fo - 13800000.f * fc - 8581568.f)
+        - float(2. * Phys::ko) * lapo;
+    muCr = (1.f - hh) * (1166528.60789006f * fc + 3200000.f * fo) + hh * (14000000.f * fc - 13800000.f * fo + 2777045.64f)
+         - float(2. * Phys::kc) * lapc;
+}
+
+// o, cr at the 7 points of a cell from global memory, with the inlet substitution.
+template <int DIM>
+__device__ __forceinline__ void gather_oc(const Par& P, const double* o, const double* cr, const Nbr& n, bool enf,
+                                          double* o7, double* c7) {
+    const long long idx[7] = {n.c, n.xm, n.xp, n.ym, n.yp, n.zm, n.zp};
+    const int xm = n.x == 0 ? 0 : n.x - 1, xp = n.x == P.nx - 1 ? n.x : n.x + 1;
+    const int qx[7] = {n.x, xm, xp, n.x, n.x, n.x, n.x};
+#pragma unroll
+    for (int s = 0; s < 7; ++s) {
+        if (DIM == 2 && s >= SZM) {
+            o7[s] = o7[SC];
+            c7[s] = c7[SC];
+            continue;
+        }
+        o7[s] = (enf && qx[s] == 0) ? Phys::layerO : o[idx[s]];
+        c7[s] = cr[idx[s]];
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void mu64_at(const Par& P, const double* o, const double* cr, const double* p,
+                                         const Nbr& n, bool enforce, double& muO, double& muCr) {
+    double o7[7], c7[7];
+    gather_oc<DIM>(P, o, cr, n, enforce, o7, c7);
+    mu64_vals<DIM>(P, o7, c7, p[n.c], muO, muCr);
 }
 
 template <int DIM, int PREC, class TE, class TM>
@@ -110,44 +151,17 @@ __global__ __launch_bounds__(TILE) void k_mu(Par P, Bufs<TE, TM> B) {
     const int t = xcd_remap(blockIdx.x, P.ntiles);
     if (!tile_cell(P, t, x, y, z)) return;
     const Nbr n = neighbours<DIM>(P, x, y, z);
-    const double* o = B.o[cur];
-    const double* cr = B.cr[cur];
-    const double* p = B.p[cur];
-    const bool enf = P.enforce_layer != 0;
+    double o7[7], c7[7];
+    gather_oc<DIM>(P, B.o[cur], B.cr[cur], n, P.enforce_layer != 0, o7, c7);
+    const double pc = B.p[cur][n.c];
     if constexpr (PREC == PREC_FP64) {
         double muO, muCr;
-        mu64_at<DIM>(P, o, cr, p, n, enf, muO, muCr);
+        mu64_vals<DIM>(P, o7, c7, pc, muO, muCr);
         B.mu_o[n.c] = (TM)muO;
         B.mu_cr[n.c] = (TM)muCr;
     } else {
-#pragma clang fp contract(fast)
-        const double oc = (enf && x == 0) ? Phys::layerO : o[n.c];
-        const double oxm = (enf && x <= 1) ? Phys::layerO : o[n.xm];
-        const double oxp = o[n.xp];
-        const double onb_y = (enf && x == 0) ? Phys::layerO : o[n.yp];
-        const double onb_ym = (enf && x == 0) ? Phys::layerO : o[n.ym];
-        const double cc = cr[n.c];
-        double lo2 = (oxp - 2. * oc + oxm);
-        double lo2y = (onb_y - 2. * oc + onb_ym);
-        double lc2 = (cr[n.xp] - 2. * cc + cr[n.xm]);
-        double lc2y = (cr[n.yp] - 2. * cc + cr[n.ym]);
-        float lapo = (float)lo2 * P.fdx2i + (float)lo2y * P.fdy2i;
-        float lapc = (float)lc2 * P.fdx2i + (float)lc2y * P.fdy2i;
-        if (DIM == 3) {
-            const double oz = (enf && x == 0) ? Phys::layerO : o[n.zp];
-            const double ozm = (enf && x == 0) ? Phys::layerO : o[n.zm];
-            lapo += (float)(oz - 2. * oc + ozm) * P.fdz2i;
-            lapc += (float)(cr[n.zp] - 2. * cc + cr[n.zm]) * P.fdz2i;
-        }
-        const float hh = h32((float)p[n.c]);
-        const float fo = (float)oc, fc = (float)cc;
-        // mu_O + 298432 and mu_Cr + 97045.64, expanded so the metal constant cancels.
-        const float muO = (1.f - hh) * (12000000.f * fo + 3200000.f * fc)
-                        + hh * (24000000.f * fo - 13800000.f * fc - 8581568.f)
-                        - float(2. * Phys::ko) * lapo;
-        const float muCr = (1.f - hh) * (1166528.60789006f * fc + 3200000.f * fo)
-                         + hh * (14000000.f * fc - 13800000.f * fo + 2777045.64f)
-                         - float(2. * Phys::kc) * lapc;
+        float muO, muCr;
+        mu32_vals<DIM>(P, o7, c7, pc, muO, muCr);
         B.mu_o[n.c] = (TM)muO;
         B.mu_cr[n.c] = (TM)muCr;
     }
@@ -179,6 +193,189 @@ __device__ __forceinline__ DD block_reduce_dd(DD v, DD* sh) {
 // phi update, legacy Ck, finite check and bounds (2D:430-433,435-450,553-591;
 // SL:616-638,771-809). Persistent grid over tiles; one budget partial per block.
 // ---------------------------------------------------------------------------
+// Values of one cell's 7-point stencil (o with the inlet substitution).
+template <class TM>
+struct Cell7 {
+    double o[7], cr[7], p[7], g[7];
+    TM muO[7], muCr[7];
+};
+
+// Trial values of one cell: mobilities (refresh loop 2, 2D:430-433; ghosts are
+// copies, so M(ghost) = M(copy)), div_flux with the mu-BC inlet face
+// (2D:435-450, SL:621-638), the phi force and update (2D:562-566, SL:780-784).
+template <int DIM, int PREC, class TM>
+__device__ __forceinline__ void trial_core(const Par& P, const Cell7<TM>& s, int x, double S, double& onv,
+                                           double& crv, double& pnv, double& left_flux) {
+    const double oo = s.o[SC], cc = s.cr[SC], pp = s.p[SC];
+    left_flux = 0.;
+    if constexpr (PREC == PREC_FP64) {
+        const double g_c = s.g[SC];
+        const double moc = mobO64(P, oo, pp, g_c);
+        const double mcc = mobCr64(P, cc, pp, g_c);
+        double mo[7], mc[7];
+#pragma unroll
+        for (int q = 1; q < 7; ++q) {
+            if (DIM == 2 && q >= SZM) break;
+            mo[q] = mobO64(P, s.o[q], s.p[q], s.g[q]);
+            mc[q] = mobCr64(P, s.cr[q], s.p[q], s.g[q]);
+        }
+        const double uo = (double)s.muO[SC], uc = (double)s.muCr[SC];
+        double divO = 0.;
+        if (P.oxy) {
+            const double right = -.5 * (moc + mo[SXP]) * ((double)s.muO[SXP] - uo) / P.dx;
+            double left = -.5 * (mo[SXM] + moc) * (uo - (double)s.muO[SXM]) / P.dx;
+            if (x == 0 && P.bc == 1) {
+                const double mf = mobO64(P, P.c_ref, pp, g_c);
+                left_flux = 2. * mf * (P.mu_res - uo) / P.dx;
+                left = left_flux;
+            }
+            const double top = -.5 * (moc + mo[SYP]) * ((double)s.muO[SYP] - uo) / P.dy;
+            const double bottom = -.5 * (mo[SYM] + moc) * (uo - (double)s.muO[SYM]) / P.dy;
+            double d = (left - right) / P.dx + (bottom - top) / P.dy;
+            if (DIM == 3) {
+                const double front = -.5 * (moc + mo[SZP]) * ((double)s.muO[SZP] - uo) / P.dz;
+                const double back = -.5 * (mo[SZM] + moc) * (uo - (double)s.muO[SZM]) / P.dz;
+                d = d + (back - front) / P.dz;
+            }
+            divO = d;
+        }
+        double divC;
+        {
+            const double right = -.5 * (mcc + mc[SXP]) * ((double)s.muCr[SXP] - uc) / P.dx;
+            const double left = -.5 * (mc[SXM] + mcc) * (uc - (double)s.muCr[SXM]) / P.dx;
+            const double top = -.5 * (mcc + mc[SYP]) * ((double)s.muCr[SYP] - uc) / P.dy;
+            const double bottom = -.5 * (mc[SYM] + mcc) * (uc - (double)s.muCr[SYM]) / P.dy;
+            double d = (left - right) / P.dx + (bottom - top) / P.dy;
+            if (DIM == 3) {
+                const double front = -.5 * (mcc + mc[SZP]) * ((double)s.muCr[SZP] - uc) / P.dz;
+                const double back = -.5 * (mc[SZM] + mcc) * (uc - (double)s.muCr[SZM]) / P.dz;
+                d = d + (back - front) / P.dz;
+            }
+            divC = d;
+        }
+        onv = P.oxy ? oo + P.dt * divO : oo;
+        crv = cc + P.dt * divC;
+        const double hp = 30. * pp * pp * (pp - 1.) * (pp - 1.);
+        double lp = (s.p[SXP] - 2. * pp + s.p[SXM]) / (P.dx * P.dx) + (s.p[SYP] - 2. * pp + s.p[SYM]) / (P.dy * P.dy);
+        if (DIM == 3) lp = lp + (s.p[SZP] - 2. * pp + s.p[SZM]) / (P.dz * P.dz);
+        double fp = Phys::W * (4. * pp * pp * pp - 6. * pp * pp + 2. * pp)
+                  + hp * (foxide64(oo, cc) - fmetal64(oo, cc)) - 2. * Phys::kp * lp;
+        if (P.has_eta) fp += 0.5 * hp * (P.etaW * (S - (1. - h64(pp))) + P.etaA * S);
+        pnv = pp - P.dt * Phys::L * fp;
+    } else {
+#pragma clang fp contract(fast)
+        const float hc = h32((float)pp);
+        const float g_c = (float)s.g[SC];
+        const float moc = P.oxy ? mobO32(P, (float)oo, hc, g_c) : 0.f;
+        const float mcc = mobCr32(P, (float)cc, hc, g_c);
+        const float uo = (float)s.muO[SC], uc = (float)s.muCr[SC];
+        float sumO = 0.f, sumC = 0.f;   // sum over faces of M_face * (mu_nb - mu_c) / h^2
+        auto face = [&](int q, float inv2, bool with_o) {
+            const float hq = h32((float)s.p[q]);
+            const float gq = (float)s.g[q];
+            if (P.oxy && with_o) {
+                const float mq = mobO32(P, (float)s.o[q], hq, gq);
+                sumO += 0.5f * (moc + mq) * ((float)s.muO[q] - uo) * inv2;
+            }
+            const float mq2 = mobCr32(P, (float)s.cr[q], hq, gq);
+            sumC += 0.5f * (mcc + mq2) * ((float)s.muCr[q] - uc) * inv2;
+        };
+        face(SXP, P.fdx2i, true);
+        const bool inlet = x == 0 && P.bc == 1;
+        face(SXM, P.fdx2i, !inlet);
+        if (inlet && P.oxy) {
+            // left face replaced by the binary64 inlet flux (2D:435-442,446)
+            double m64O, m64C;
+            mu64_vals<DIM>(P, s.o, s.cr, pp, m64O, m64C);
+            const double mf = mobO64(P, P.c_ref, pp, s.g[SC]);
+            left_flux = 2. * mf * (P.mu_res - m64O) / P.dx;
+            sumO += (float)(left_flux / P.dx);
+        }
+        face(SYP, P.fdy2i, true);
+        face(SYM, P.fdy2i, true);
+        if (DIM == 3) {
+            face(SZP, P.fdz2i, true);
+            face(SZM, P.fdz2i, true);
+        }
+        onv = P.oxy ? oo + P.dt * (double)sumO : oo;
+        crv = cc + P.dt * (double)sumC;
+        const float fpp = (float)pp;
+        const float hp = 30.f * fpp * fpp * (fpp - 1.f) * (fpp - 1.f);
+        const float lp = (float)(s.p[SXP] - 2. * pp + s.p[SXM]) * P.fdx2i + (float)(s.p[SYP] - 2. * pp + s.p[SYM]) * P.fdy2i
+                       + (DIM == 3 ? (float)(s.p[SZP] - 2. * pp + s.p[SZM]) * P.fdz2i : 0.f);
+        const float fo = (float)oo, fcr = (float)cc;
+        // foxide - fmetal, expanded (2D:174-181)
+        const float dfx = 6000000.f * fo * fo + (7000000.f - 583264.303945031f) * fcr * fcr - 17000000.f * fo * fcr
+                        - 8581568.f * fo + 2777045.64f * fcr + float(1860990. + 61732.2974223055);
+        float fp = float(Phys::W) * (4.f * fpp * fpp * fpp - 6.f * fpp * fpp + 2.f * fpp) + hp * dfx
+                 - float(2. * Phys::kp) * lp;
+        if (P.has_eta) fp += 0.5f * hp * ((float)P.etaW * ((float)S - (1.f - hc)) + (float)P.etaA * (float)S);
+        pnv = pp - P.dt * Phys::L * (double)fp;
+    }
+}
+
+// Per-block accumulators of the step budget.
+struct StepAcc {
+    DD bc{0., 0.}, ckO{0., 0.}, ckCr{0., 0.}, clO{0., 0.}, clCr{0., 0.};
+    double n_clip = 0.;
+    unsigned long long bad = NO_KEY;
+};
+
+// Legacy Ck (2D:567-571), finite check (2D:572-573), bounds: clamp + phi snap
+// (2D:574-580) or rejection (2D:581-589), inlet budget (2D:408-411,553).
+// binary64 in both precisions. Writes the trial values (raw at a failing cell).
+__device__ __forceinline__ void finish_cell(const Par& P, long long c, int x, double o_stored, double oo, double cc,
+                                            double pp, double onv, double crv, double pnv, double left_flux,
+                                            StepAcc& a, double* on, double* crn, double* pn) {
+    if (P.ck_legacy) {
+        double dO = -Phys::ckO * ((Phys::oxideO - onv) * h64(pnv) - (Phys::oxideO - oo) * h64(pp)) / P.dt;
+        double dCr = -Phys::ckCr * ((Phys::oxideCr - crv) * h64(pnv) - (Phys::oxideCr - cc) * h64(pp)) / P.dt;
+        onv += dO;
+        crv += dCr;
+        a.ckO = dd_add(a.ckO, dO);
+        a.ckCr = dd_add(a.ckCr, dCr);
+    }
+    if (x == 0) {
+        if (P.enforce_layer) a.bc = dd_add(a.bc, Phys::layerO - o_stored);
+        if (P.bc == 1 && P.oxy) a.bc = dd_add(a.bc, P.dt * left_flux / P.dx);
+    }
+    if (!isfinite(onv) || !isfinite(crv) || !isfinite(pnv)) {
+        a.bad = a.bad < (unsigned long long)c ? a.bad : (unsigned long long)c;
+    } else if (P.bounds_legacy) {
+        const double oc = onv < 0. ? 0. : (1. < onv ? 1. : onv);
+        const double ccr = crv < 0. ? 0. : (1. < crv ? 1. : crv);
+        a.clO = dd_add(a.clO, oc - onv);
+        a.clCr = dd_add(a.clCr, ccr - crv);
+        if (oc != onv || ccr != crv) a.n_clip += 1.;
+        onv = oc;
+        crv = ccr;
+        if (pnv > .99999) pnv = 1.;
+        if (pnv < 1.e-5) pnv = 0.;
+    } else if (onv < -1.e-12 || crv < -1.e-12 || onv + crv > 1. + 1.e-12 || pnv < -1.e-12 || pnv > 1. + 1.e-12) {
+        a.bad = a.bad < (unsigned long long)c ? a.bad : (unsigned long long)c;
+    }
+    on[c] = onv;
+    crn[c] = crv;
+    pn[c] = pnv;
+}
+
+// Block-reduce the accumulators into this block's partial slot.
+__device__ __forceinline__ void write_partials(const StepAcc& a, double* partials, DevCtl* ctl) {
+    if (a.bad != NO_KEY) atomicMin(&ctl->pde_key, a.bad);
+    __shared__ DD sh[16];
+    const DD r0 = block_reduce_dd(a.bc, sh);
+    const DD r1 = block_reduce_dd(a.ckO, sh);
+    const DD r2 = block_reduce_dd(a.ckCr, sh);
+    const DD r3 = block_reduce_dd(a.clO, sh);
+    const DD r4 = block_reduce_dd(a.clCr, sh);
+    const DD r5 = block_reduce_dd(DD{a.n_clip, 0.}, sh);
+    if (threadIdx.x == 0) {
+        double* w = partials + (long long)blockIdx.x * NPART;
+        w[0] = r0.hi; w[1] = r0.lo; w[2] = r1.hi; w[3] = r1.lo; w[4] = r2.hi; w[5] = r2.lo;
+        w[6] = r3.hi; w[7] = r3.lo; w[8] = r4.hi; w[9] = r4.lo; w[10] = r5.hi; w[11] = 0.;
+    }
+}
+
 template <int DIM, int PREC, class TE, class TM>
 __global__ __launch_bounds__(TILE) void k_trial(Par P, Bufs<TE, TM> B) {
     DevCtl* ctl = B.ctl;
@@ -187,212 +384,259 @@ __global__ __launch_bounds__(TILE) void k_trial(Par P, Bufs<TE, TM> B) {
     const double* o = B.o[cur];
     const double* cr = B.cr[cur];
     const double* p = B.p[cur];
-    double* on = B.o[nxt];
-    double* crn = B.cr[nxt];
-    double* pn = B.p[nxt];
     const int me = P.eta_evolves ? cur : 0;
     const TE* gb = B.gb[me];
     const TE* s2 = B.s2[me];
-    const TM* muO = B.mu_o;
-    const TM* muCr = B.mu_cr;
     const bool enf = P.enforce_layer != 0;
-
-    DD a_bc{0., 0.}, a_ckO{0., 0.}, a_ckCr{0., 0.}, a_clO{0., 0.}, a_clCr{0., 0.};
-    double n_clip = 0.;
-    unsigned long long bad = NO_KEY;
-
+    StepAcc acc;
     const int G = gridDim.x;
-    for (int vb = blockIdx.x; vb < ((P.ntiles + G - 1) / G) * G; vb += G) {
-        if (vb >= P.ntiles) break;
+    for (int vb = blockIdx.x; vb < P.ntiles; vb += G) {
         const int t = xcd_remap(vb, P.ntiles);
         int x, y, z;
         if (!tile_cell(P, t, x, y, z)) continue;
         const Nbr n = neighbours<DIM>(P, x, y, z);
         const long long c = n.c;
-        const double o_stored = o[c];
-        const double oo = (enf && x == 0) ? Phys::layerO : o_stored;
-        const double cc = cr[c], pp = p[c];
-        double onv, crv, pnv;
-        double left_flux = 0.;
-        if constexpr (PREC == PREC_FP64) {
-            // o with the inlet substitution, at the 7 stencil points
-            auto O = [&](long long q, int qx) { return (enf && qx == 0) ? Phys::layerO : o[q]; };
-            const int xm = x == 0 ? 0 : x - 1, xp = x == P.nx - 1 ? x : x + 1;
-            const double o_xm = O(n.xm, xm), o_xp = O(n.xp, xp), o_ym = O(n.ym, x), o_yp = O(n.yp, x);
-            const double o_zm = DIM == 3 ? O(n.zm, x) : 0., o_zp = DIM == 3 ? O(n.zp, x) : 0.;
-            const double g_c = (double)gb[c];
-            // mobilities (refresh loop 2, 2D:430-433): ghosts are copies, so M(ghost) = M(copy)
-            const double moc = mobO64(P, oo, pp, g_c);
-            const double mcc = mobCr64(P, cc, pp, g_c);
-            const double mo_xm = mobO64(P, o_xm, p[n.xm], (double)gb[n.xm]);
-            const double mo_xp = mobO64(P, o_xp, p[n.xp], (double)gb[n.xp]);
-            const double mo_ym = mobO64(P, o_ym, p[n.ym], (double)gb[n.ym]);
-            const double mo_yp = mobO64(P, o_yp, p[n.yp], (double)gb[n.yp]);
-            const double mc_xm = mobCr64(P, cr[n.xm], p[n.xm], (double)gb[n.xm]);
-            const double mc_xp = mobCr64(P, cr[n.xp], p[n.xp], (double)gb[n.xp]);
-            const double mc_ym = mobCr64(P, cr[n.ym], p[n.ym], (double)gb[n.ym]);
-            const double mc_yp = mobCr64(P, cr[n.yp], p[n.yp], (double)gb[n.yp]);
-            const double uo = (double)muO[c], uc = (double)muCr[c];
-            // div_flux (2D:443-450, SL:629-638)
-            double divO = 0.;
-            if (P.oxy) {
-                const double right = -.5 * (moc + mo_xp) * ((double)muO[n.xp] - uo) / P.dx;
-                double left = -.5 * (mo_xm + moc) * (uo - (double)muO[n.xm]) / P.dx;
-                if (x == 0 && P.bc == 1) {
-                    // left_flux (2D:435-442): positive face-mobility closure
-                    const double mf = mobO64(P, P.c_ref, pp, g_c);
-                    left_flux = 2. * mf * (P.mu_res - uo) / P.dx;
-                    left = left_flux;
-                }
-                const double top = -.5 * (moc + mo_yp) * ((double)muO[n.yp] - uo) / P.dy;
-                const double bottom = -.5 * (mo_ym + moc) * (uo - (double)muO[n.ym]) / P.dy;
-                double d = (left - right) / P.dx + (bottom - top) / P.dy;
-                if (DIM == 3) {
-                    const double mo_zm = mobO64(P, o_zm, p[n.zm], (double)gb[n.zm]);
-                    const double mo_zp = mobO64(P, o_zp, p[n.zp], (double)gb[n.zp]);
-                    const double front = -.5 * (moc + mo_zp) * ((double)muO[n.zp] - uo) / P.dz;
-                    const double back = -.5 * (mo_zm + moc) * (uo - (double)muO[n.zm]) / P.dz;
-                    d = d + (back - front) / P.dz;
-                }
-                divO = d;
-            }
-            double divC;
-            {
-                const double right = -.5 * (mcc + mc_xp) * ((double)muCr[n.xp] - uc) / P.dx;
-                const double left = -.5 * (mc_xm + mcc) * (uc - (double)muCr[n.xm]) / P.dx;
-                const double top = -.5 * (mcc + mc_yp) * ((double)muCr[n.yp] - uc) / P.dy;
-                const double bottom = -.5 * (mc_ym + mcc) * (uc - (double)muCr[n.ym]) / P.dy;
-                double d = (left - right) / P.dx + (bottom - top) / P.dy;
-                if (DIM == 3) {
-                    const double mc_zm = mobCr64(P, cr[n.zm], p[n.zm], (double)gb[n.zm]);
-                    const double mc_zp = mobCr64(P, cr[n.zp], p[n.zp], (double)gb[n.zp]);
-                    const double front = -.5 * (mcc + mc_zp) * ((double)muCr[n.zp] - uc) / P.dz;
-                    const double back = -.5 * (mc_zm + mcc) * (uc - (double)muCr[n.zm]) / P.dz;
-                    d = d + (back - front) / P.dz;
-                }
-                divC = d;
-            }
-            onv = P.oxy ? oo + P.dt * divO : oo;
-            crv = cc + P.dt * divC;
-            const double hp = 30. * pp * pp * (pp - 1.) * (pp - 1.);
-            const double lp = lap64<DIM>(p, n, pp, P, p[n.xm], p[n.xp]);
-            double fp = Phys::W * (4. * pp * pp * pp - 6. * pp * pp + 2. * pp)
-                      + hp * (foxide64(oo, cc) - fmetal64(oo, cc)) - 2. * Phys::kp * lp;
-            if (P.has_eta) {
-                const double S = (double)s2[c];
-                const double hpp = 30. * pp * pp * (pp - 1.) * (pp - 1.);
-                fp += 0.5 * hpp * (P.etaW * (S - (1. - h64(pp))) + P.etaA * S);
-            }
-            pnv = pp - P.dt * Phys::L * fp;
-        } else {
-#pragma clang fp contract(fast)
-            const int xm = x == 0 ? 0 : x - 1;
-            auto O = [&](long long q, int qx) { return (enf && qx == 0) ? Phys::layerO : o[q]; };
-            const float hc = h32((float)pp);
-            const float g_c = (float)gb[c];
-            const float moc = P.oxy ? mobO32(P, (float)oo, hc, g_c) : 0.f;
-            const float mcc = mobCr32(P, (float)cc, hc, g_c);
-            const float uo = (float)muO[c], uc = (float)muCr[c];
-            float sumO = 0.f, sumC = 0.f;   // sum over faces of M_face * (mu_nb - mu_c)
-            auto face = [&](long long q, int qx, float inv2) {
-                const float hq = h32((float)p[q]);
-                const float gq = (float)gb[q];
-                if (P.oxy) {
-                    const float mq = mobO32(P, (float)O(q, qx), hq, gq);
-                    sumO += 0.5f * (moc + mq) * ((float)muO[q] - uo) * inv2;
-                }
-                const float mq2 = mobCr32(P, (float)cr[q], hq, gq);
-                sumC += 0.5f * (mcc + mq2) * ((float)muCr[q] - uc) * inv2;
-            };
-            face(n.xp, x == P.nx - 1 ? x : x + 1, P.fdx2i);
-            if (!(x == 0 && P.bc == 1)) {
-                face(n.xm, xm, P.fdx2i);
+        Cell7<TM> s;
+        gather_oc<DIM>(P, o, cr, n, enf, s.o, s.cr);
+        const long long idx[7] = {n.c, n.xm, n.xp, n.ym, n.yp, n.zm, n.zp};
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            s.p[q] = p[idx[q]];
+            s.g[q] = (double)gb[idx[q]];
+            s.muO[q] = B.mu_o[idx[q]];
+            s.muCr[q] = B.mu_cr[idx[q]];
+        }
+        const double S = P.has_eta ? (double)s2[c] : 0.;
+        double onv, crv, pnv, left_flux;
+        trial_core<DIM, PREC, TM>(P, s, x, S, onv, crv, pnv, left_flux);
+        finish_cell(P, c, x, o[c], s.o[SC], s.cr[SC], s.p[SC], onv, crv, pnv, left_flux, acc, B.o[nxt], B.cr[nxt],
+                    B.p[nxt]);
+    }
+    write_partials(acc, B.partials, ctl);
+}
+
+// ---------------------------------------------------------------------------
+// k_pde_march (3D): k_mu and k_trial fused, so mu never makes an HBM round trip
+// and every o, cr, phi, gb value is fetched from HBM once. A block owns a 64 (z)
+// x TY (y) column tile and marches along X over XS planes:
+//   LDS ring of 4 planes of o, cr (two-cell Y/Z halo) and phi, gb (one-cell halo),
+//   LDS ring of 2 planes of mu_O, mu_Cr (one-cell halo),
+//   own-cell values of plane x-1 in registers.
+// Iteration x: load plane x+2 | sync | mu(x+1) over tile+halo | sync | trial(x).
+// The arithmetic is trial_core / mu*_vals, shared with k_mu + k_trial, so the
+// binary64 results are bitwise equal to theirs.
+// ---------------------------------------------------------------------------
+template <int PREC, int TY>
+struct PdeLds {
+    static constexpr int W2 = 68, H2 = TY + 4, W1 = 66, H1 = TY + 2;
+    static constexpr int P2 = H2 * W2, P1 = H1 * W1;
+    using TMU = typename std::conditional<PREC == PREC_FP64, double, float>::type;
+    using TG = TMU;
+    static constexpr size_t bytes() {
+        return size_t(4) * P2 * 8 * 2 + size_t(4) * P1 * 8 + size_t(4) * P1 * sizeof(TG) + size_t(2) * P1 * sizeof(TMU) * 2;
+    }
+};
+
+template <int PREC, int TY, class TE, class TM>
+__global__ __launch_bounds__(64 * TY) void k_pde_march(Par P, Bufs<TE, TM> B, int XS) {
+    using L = PdeLds<PREC, TY>;
+    using TMU = typename L::TMU;
+    using TG = typename L::TG;
+    extern __shared__ unsigned char lds_raw[];
+    double* so = reinterpret_cast<double*>(lds_raw);             // [4][P2]
+    double* sc = so + 4 * L::P2;                                  // [4][P2]
+    double* sp = sc + 4 * L::P2;                                  // [4][P1]
+    TG* sg = reinterpret_cast<TG*>(sp + 4 * L::P1);               // [4][P1]
+    TMU* smo = reinterpret_cast<TMU*>(sg + 4 * L::P1);            // [2][P1]
+    TMU* smc = smo + 2 * L::P1;                                   // [2][P1]
+    DevCtl* ctl = B.ctl;
+    if (ctl->failed) return;
+    const int cur = ctl->cur, nxt = cur ^ 1;
+    const int nth = 64 * TY;
+    const int tz = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int nTz = (P.nz + 63) >> 6, nTy = (P.ny + TY - 1) / TY, nSeg = (P.nx + XS - 1) / XS;
+    const int b = xcd_remap(blockIdx.x, nTz * nTy * nSeg);
+    const int seg = b / (nTz * nTy);
+    const int rem = b - seg * (nTz * nTy);
+    const int tyT = rem / nTz, tzT = rem - tyT * nTz;
+    const int z0 = tzT * 64, y0 = tyT * TY;
+    const int y = y0 + ty, z = z0 + tz;
+    const bool active = y < P.ny && z < P.nz;
+    const int x0 = seg * XS, x1 = min(P.nx, x0 + XS);
+    const long long plane = P.nyz;
+    const double* o = B.o[cur];
+    const double* cr = B.cr[cur];
+    const double* p = B.p[cur];
+    const int me = P.eta_evolves ? cur : 0;
+    const TE* gb = B.gb[me];
+    const TE* s2 = B.s2[me];
+    const bool enf = P.enforce_layer != 0;
+    auto wrapy = [&](int v) { return ((v % P.ny) + P.ny) % P.ny; };
+    auto wrapz = [&](int v) { return ((v % P.nz) + P.nz) % P.nz; };
+    auto clampx = [&](int v) { return v < 0 ? 0 : (v >= P.nx ? P.nx - 1 : v); };
+
+    // plane X (clamped: the X ghosts are copies) into ring slot X & 3
+    auto load_plane = [&](int X) {
+        const int xc = clampx(X), slot = X & 3;
+        const long long xo = (long long)xc * plane;
+        const bool sub = enf && xc == 0;
+        for (int q = threadIdx.x; q < L::P2; q += nth) {
+            const int ly = q / L::W2, lz = q - ly * L::W2;
+            const long long g = xo + (long long)wrapy(y0 - 2 + ly) * P.nz + wrapz(z0 - 2 + lz);
+            so[slot * L::P2 + q] = sub ? Phys::layerO : o[g];
+            sc[slot * L::P2 + q] = cr[g];
+        }
+        for (int q = threadIdx.x; q < L::P1; q += nth) {
+            const int ly = q / L::W1, lz = q - ly * L::W1;
+            const long long g = xo + (long long)wrapy(y0 - 1 + ly) * P.nz + wrapz(z0 - 1 + lz);
+            sp[slot * L::P1 + q] = p[g];
+            sg[slot * L::P1 + q] = (TG)gb[g];
+        }
+    };
+    // 7-point o, cr around halo-1 cell (ly1, lz1) of plane X from the rings
+    // (with_xm false: the x-1 entries are left for the caller to fill)
+    auto oc7 = [&](int X, int ly1, int lz1, double* o7, double* c7, bool with_xm) {
+        const int s = X & 3, sm = (X - 1) & 3, sp_ = (X + 1) & 3;
+        const int q = (ly1 + 1) * L::W2 + (lz1 + 1);
+        o7[SC] = so[s * L::P2 + q];
+        c7[SC] = sc[s * L::P2 + q];
+        if (with_xm) {
+            o7[SXM] = so[sm * L::P2 + q];
+            c7[SXM] = sc[sm * L::P2 + q];
+        }
+        o7[SXP] = so[sp_ * L::P2 + q];
+        c7[SXP] = sc[sp_ * L::P2 + q];
+        o7[SYM] = so[s * L::P2 + q - L::W2];
+        c7[SYM] = sc[s * L::P2 + q - L::W2];
+        o7[SYP] = so[s * L::P2 + q + L::W2];
+        c7[SYP] = sc[s * L::P2 + q + L::W2];
+        o7[SZM] = so[s * L::P2 + q - 1];
+        c7[SZM] = sc[s * L::P2 + q - 1];
+        o7[SZP] = so[s * L::P2 + q + 1];
+        c7[SZP] = sc[s * L::P2 + q + 1];
+    };
+    // mu of plane X over the tile and its one-cell halo, into mu ring slot X & 1
+    auto mu_plane = [&](int X) {
+        const int ms = X & 1;
+        for (int q = threadIdx.x; q < L::P1; q += nth) {
+            const int ly1 = q / L::W1, lz1 = q - ly1 * L::W1;
+            double o7[7], c7[7];
+            oc7(X, ly1, lz1, o7, c7, true);
+            const double pc = sp[(X & 3) * L::P1 + q];
+            if constexpr (PREC == PREC_FP64) {
+                double a, bb;
+                mu64_vals<3>(P, o7, c7, pc, a, bb);
+                smo[ms * L::P1 + q] = a;
+                smc[ms * L::P1 + q] = bb;
             } else {
-                // left face replaced by the binary64 inlet flux (2D:435-442,446)
-                const float hq = h32((float)p[n.xm]);
-                const float mq2 = mobCr32(P, (float)cr[n.xm], hq, (float)gb[n.xm]);
-                sumC += 0.5f * (mcc + mq2) * ((float)muCr[n.xm] - uc) * P.fdx2i;
-                if (P.oxy) {
-                    double m64O, m64C;
-                    mu64_at<DIM>(P, o, cr, p, n, enf, m64O, m64C);
-                    const double mf = mobO64(P, P.c_ref, pp, (double)g_c);
-                    left_flux = 2. * mf * (P.mu_res - m64O) / P.dx;
-                    sumO += (float)(left_flux / P.dx);
-                }
+                float a, bb;
+                mu32_vals<3>(P, o7, c7, pc, a, bb);
+                smo[ms * L::P1 + q] = a;
+                smc[ms * L::P1 + q] = bb;
             }
-            face(n.yp, x, P.fdy2i);
-            face(n.ym, x, P.fdy2i);
-            if (DIM == 3) {
-                face(n.zp, x, P.fdz2i);
-                face(n.zm, x, P.fdz2i);
+        }
+    };
+
+    // prologue: planes x0-1, x0, x0+1; mu(x0); own-cell values of plane x0-1
+    load_plane(x0 - 1);
+    load_plane(x0);
+    load_plane(x0 + 1);
+    __syncthreads();
+    mu_plane(x0);
+    __syncthreads();
+    const int own1 = (ty + 1) * L::W1 + (tz + 1);
+    const int own2 = (ty + 2) * L::W2 + (tz + 2);
+    double o_prev, c_prev, p_prev;
+    TG g_prev;
+    TMU mo_prev, mc_prev;
+    {
+        const int s = (x0 - 1) & 3;
+        o_prev = so[s * L::P2 + own2];
+        c_prev = sc[s * L::P2 + own2];
+        p_prev = sp[s * L::P1 + own1];
+        g_prev = sg[s * L::P1 + own1];
+        if (x0 == 0) {
+            // the x-1 ghost of the first plane is a copy of it, mu included (2D:427)
+            mo_prev = smo[(x0 & 1) * L::P1 + own1];
+            mc_prev = smc[(x0 & 1) * L::P1 + own1];
+        } else {
+            // mu(x0-1) at the own cell: its x-1 neighbour is plane x0-2 (global)
+            double o7[7], c7[7];
+            oc7(x0 - 1, ty + 1, tz + 1, o7, c7, false);
+            const int xmm = clampx(x0 - 2);
+            const long long gm = (long long)xmm * plane + (long long)wrapy(y) * P.nz + wrapz(z);
+            o7[SXM] = (enf && xmm == 0) ? Phys::layerO : o[gm];
+            c7[SXM] = cr[gm];
+            if constexpr (PREC == PREC_FP64) {
+                double a, bb;
+                mu64_vals<3>(P, o7, c7, p_prev, a, bb);
+                mo_prev = a;
+                mc_prev = bb;
+            } else {
+                float a, bb;
+                mu32_vals<3>(P, o7, c7, p_prev, a, bb);
+                mo_prev = a;
+                mc_prev = bb;
             }
-            onv = P.oxy ? oo + P.dt * (double)sumO : oo;
-            crv = cc + P.dt * (double)sumC;
-            const float fpp = (float)pp;
-            const float hp = 30.f * fpp * fpp * (fpp - 1.f) * (fpp - 1.f);
-            const float lp = (float)(p[n.xp] - 2. * pp + p[n.xm]) * P.fdx2i
-                           + (float)(p[n.yp] - 2. * pp + p[n.ym]) * P.fdy2i
-                           + (DIM == 3 ? (float)(p[n.zp] - 2. * pp + p[n.zm]) * P.fdz2i : 0.f);
-            const float fo = (float)oo, fcr = (float)cc;
-            // foxide - fmetal, expanded (2D:174-181)
-            const float dfx = 6000000.f * fo * fo + (7000000.f - 583264.303945031f) * fcr * fcr
-                            - 17000000.f * fo * fcr - 8581568.f * fo + 2777045.64f * fcr
-                            + float(1860990. + 61732.2974223055);
-            float fp = float(Phys::W) * (4.f * fpp * fpp * fpp - 6.f * fpp * fpp + 2.f * fpp)
-                     + hp * dfx - float(2. * Phys::kp) * lp;
-            if (P.has_eta) {
-                const float S = (float)s2[c];
-                fp += 0.5f * hp * ((float)P.etaW * (S - (1.f - hc)) + (float)P.etaA * S);
-            }
-            pnv = pp - P.dt * Phys::L * (double)fp;
         }
-        // legacy Ck (2D:567-571), binary64 in both precisions
-        if (P.ck_legacy) {
-            double dO = -Phys::ckO * ((Phys::oxideO - onv) * h64(pnv) - (Phys::oxideO - oo) * h64(pp)) / P.dt;
-            double dCr = -Phys::ckCr * ((Phys::oxideCr - crv) * h64(pnv) - (Phys::oxideCr - cc) * h64(pp)) / P.dt;
-            onv += dO;
-            crv += dCr;
-            a_ckO = dd_add(a_ckO, dO);
-            a_ckCr = dd_add(a_ckCr, dCr);
-        }
-        if (x == 0) {
-            if (enf) a_bc = dd_add(a_bc, Phys::layerO - o_stored);
-            if (P.bc == 1 && P.oxy) a_bc = dd_add(a_bc, P.dt * left_flux / P.dx);
-        }
-        if (!isfinite(onv) || !isfinite(crv) || !isfinite(pnv)) {
-            bad = bad < (unsigned long long)c ? bad : (unsigned long long)c;
-        } else if (P.bounds_legacy) {
-            // std::clamp then phi snap (2D:574-580)
-            const double oc = onv < 0. ? 0. : (1. < onv ? 1. : onv);
-            const double ccr = crv < 0. ? 0. : (1. < crv ? 1. : crv);
-            a_clO = dd_add(a_clO, oc - onv);
-            a_clCr = dd_add(a_clCr, ccr - crv);
-            if (oc != onv || ccr != crv) n_clip += 1.;
-            onv = oc;
-            crv = ccr;
-            if (pnv > .99999) pnv = 1.;
-            if (pnv < 1.e-5) pnv = 0.;
-        } else if (onv < -1.e-12 || crv < -1.e-12 || onv + crv > 1. + 1.e-12 ||
-                   pnv < -1.e-12 || pnv > 1. + 1.e-12) {
-            bad = bad < (unsigned long long)c ? bad : (unsigned long long)c;
-        }
-        on[c] = onv;
-        crn[c] = crv;
-        pn[c] = pnv;
     }
-    if (bad != NO_KEY) atomicMin(&ctl->pde_key, bad);
-    __shared__ DD sh[TILE / 64];
-    const DD r0 = block_reduce_dd(a_bc, sh);
-    const DD r1 = block_reduce_dd(a_ckO, sh);
-    const DD r2 = block_reduce_dd(a_ckCr, sh);
-    const DD r3 = block_reduce_dd(a_clO, sh);
-    const DD r4 = block_reduce_dd(a_clCr, sh);
-    const DD r5 = block_reduce_dd(DD{n_clip, 0.}, sh);
-    if (threadIdx.x == 0) {
-        double* w = B.partials + (long long)blockIdx.x * NPART;
-        w[0] = r0.hi; w[1] = r0.lo; w[2] = r1.hi; w[3] = r1.lo; w[4] = r2.hi; w[5] = r2.lo;
-        w[6] = r3.hi; w[7] = r3.lo; w[8] = r4.hi; w[9] = r4.lo; w[10] = r5.hi; w[11] = 0.;
+    StepAcc acc;
+    for (int x = x0; x < x1; ++x) {
+        load_plane(x + 2);
+        __syncthreads();
+        mu_plane(x + 1);
+        __syncthreads();
+        const int s = x & 3, s1 = (x + 1) & 3, ms = x & 1, ms1 = (x + 1) & 1;
+        Cell7<TMU> st;
+        oc7(x, ty + 1, tz + 1, st.o, st.cr, false);
+        st.o[SXM] = o_prev;
+        st.cr[SXM] = c_prev;
+        st.p[SC] = sp[s * L::P1 + own1];
+        st.p[SXM] = p_prev;
+        st.p[SXP] = sp[s1 * L::P1 + own1];
+        st.p[SYM] = sp[s * L::P1 + own1 - L::W1];
+        st.p[SYP] = sp[s * L::P1 + own1 + L::W1];
+        st.p[SZM] = sp[s * L::P1 + own1 - 1];
+        st.p[SZP] = sp[s * L::P1 + own1 + 1];
+        st.g[SC] = (double)sg[s * L::P1 + own1];
+        st.g[SXM] = (double)g_prev;
+        st.g[SXP] = (double)sg[s1 * L::P1 + own1];
+        st.g[SYM] = (double)sg[s * L::P1 + own1 - L::W1];
+        st.g[SYP] = (double)sg[s * L::P1 + own1 + L::W1];
+        st.g[SZM] = (double)sg[s * L::P1 + own1 - 1];
+        st.g[SZP] = (double)sg[s * L::P1 + own1 + 1];
+        st.muO[SC] = smo[ms * L::P1 + own1];
+        st.muCr[SC] = smc[ms * L::P1 + own1];
+        st.muO[SXM] = mo_prev;
+        st.muCr[SXM] = mc_prev;
+        // the x+1 ghost of the last plane is a copy of it, mu included
+        st.muO[SXP] = x + 1 == P.nx ? st.muO[SC] : smo[ms1 * L::P1 + own1];
+        st.muCr[SXP] = x + 1 == P.nx ? st.muCr[SC] : smc[ms1 * L::P1 + own1];
+        st.muO[SYM] = smo[ms * L::P1 + own1 - L::W1];
+        st.muCr[SYM] = smc[ms * L::P1 + own1 - L::W1];
+        st.muO[SYP] = smo[ms * L::P1 + own1 + L::W1];
+        st.muCr[SYP] = smc[ms * L::P1 + own1 + L::W1];
+        st.muO[SZM] = smo[ms * L::P1 + own1 - 1];
+        st.muCr[SZM] = smc[ms * L::P1 + own1 - 1];
+        st.muO[SZP] = smo[ms * L::P1 + own1 + 1];
+        st.muCr[SZP] = smc[ms * L::P1 + own1 + 1];
+        if (active) {
+            const long long c = (long long)x * plane + (long long)y * P.nz + z;
+            const double S = P.has_eta ? (double)s2[c] : 0.;
+            const double o_stored = x == 0 ? o[c] : st.o[SC];
+            double onv, crv, pnv, left_flux;
+            trial_core<3, PREC, TMU>(P, st, x, S, onv, crv, pnv, left_flux);
+            finish_cell(P, c, x, o_stored, st.o[SC], st.cr[SC], st.p[SC], onv, crv, pnv, left_flux, acc, B.o[nxt],
+                        B.cr[nxt], B.p[nxt]);
+        }
+        o_prev = st.o[SC];
+        c_prev = st.cr[SC];
+        p_prev = st.p[SC];
+        g_prev = sg[s * L::P1 + own1];
+        mo_prev = st.muO[SC];
+        mc_prev = st.muCr[SC];
     }
+    write_partials(acc, B.partials, ctl);
 }
 
 // ---------------------------------------------------------------------------

@@ -81,6 +81,11 @@ CASES = [
     (3, (9, 4, 33), 2, True, dict(bc="closed", bounds="legacy", ck="off")),
     (3, (7, 3, 40), 9, True, dict(bc="concentration", bounds="legacy", dt=2e-10)),
     (3, (40, 5, 130), 32, True, dict(bc="mu", bounds="legacy")),
+    # nz >= 32 also selects the fused X-marching PDE kernel
+    (3, (12, 9, 40), 0, True, dict(bc="concentration", bounds="legacy", dt=2e-10)),
+    (3, (8, 7, 64), 0, False, dict(bc="mu", bounds="legacy", ck="off")),
+    (3, (5, 3, 33), 0, True, dict(bc="closed", bounds="legacy", ck="legacy", dt=2e-10)),
+    (3, (21, 11, 96), 3, True, dict(bc="concentration", bounds="legacy", oxygen_diffusion=False)),
 ]
 
 
@@ -228,3 +233,21 @@ def test_march_equals_basic_kernel(nicr, shape, n_eta, monkeypatch):
                 assert np.array_equal(a, b)
             else:
                 assert np.max(np.abs(a - b)) <= 1e-6 * (1 + np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("dim,shape", [(3, (13, 9, 40))])
+def test_stop_bounds_rejection_marching_kernel(nicr, oracle, dim, shape):
+    """Rejection through the fused X-marching kernel (nz >= 32): same cell, message and state."""
+    gpu, ora = _pair(nicr, oracle, dim, shape, 0, True, bc="concentration", bounds="stop")
+    cr = gpu.get("cr")
+    cr[shape[0] - 2, 3, 30] = -0.1
+    cr[shape[0] - 1, 1, 2] = -0.2
+    gpu.set("cr", cr)
+    ora.set("cr", cr)
+    o_before = gpu.get("o")
+    rg = gpu.try_step(3)
+    ro = ora.try_step(3)
+    assert rg.status == ro.status == nicr.STEP_OUT_OF_BOUNDS
+    assert nicr.step_error_message(rg, dim) == nicr.step_error_message(ro, dim)
+    _compare(gpu, ora, "after rejection", refresh=False)
+    assert np.array_equal(gpu.get("o"), o_before)
