@@ -45,6 +45,8 @@ struct Ctx {
     int dim = 3, prec = PREC_FP64;
     long long ncell = 0;
     size_t te = 8, tm = 8;  // element sizes of eta/mask and mu scratch
+    char* slab = nullptr;   // state arrays (see nicr_create)
+    size_t slab_bytes = 0;
     hipStream_t stream = nullptr;
     double* o[2] = {nullptr, nullptr};
     double* cr[2] = {nullptr, nullptr};
@@ -67,7 +69,7 @@ struct Ctx {
     int* nuc_centres = nullptr;
     int g_trial = 0, g_eta = 0, g_stat = 0;
     int partial_slots = 0;         // budget partial slots allocated (one per block of the trial kernel)
-    int pde_ty = 4;                // Y tile height of the marching PDE kernel (NICR_PDE_TY: 2, 4)
+    int pde_ty = 4;                // Y tile height of the marching PDE kernel (NICR_PDE_TY: 2, 4, 6, 8)
     int pde_nseg = 0;              // X segments of the marching PDE kernel (NICR_PDE_NSEG; 0: auto)
     int cur = 0;                 // host mirror of ctl->cur
     bool mask_dirty = true;
@@ -151,9 +153,31 @@ void launch_mask(Ctx* c) {
     hipLaunchKernelGGL((k_mask<DIM, PREC, TE, TM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P,
                        bufs<TE, TM>(c), c->eta_evolves() ? -1 : 0);
 }
-// X-marching eta kernel for 3D meshes with up to 32 eta fields.
+// X segments per column tile for a marching kernel: every block marches nx/nseg
+// planes after a prologue of about `lead` planes, and the blocks run in
+// ceil(tiles*nseg/resident) waves; pick the nseg with the least planes per CU slot.
+static int pick_nseg(int tiles, int resident, int nx, double lead) {
+    int best = 1;
+    double best_cost = 1e300;
+    const int hi = std::min(nx, std::max(1, 8 * resident / std::max(1, tiles) + 1));
+    for (int nseg = 1; nseg <= hi; ++nseg) {
+        const int xs = (nx + nseg - 1) / nseg;
+        const int ns = (nx + xs - 1) / xs;
+        const long long blocks = (long long)tiles * ns;
+        const double waves = (double)((blocks + resident - 1) / resident);
+        const double cost = waves * (xs + lead);
+        if (cost < best_cost - 1e-9) {
+            best_cost = cost;
+            best = ns;
+        }
+    }
+    return best;
+}
+
+// X-marching eta kernel for 3D meshes with up to 32 eta fields: one pass over
+// fields g0 .. g0+NG-1.
 template <int PREC, int NG, int TY, class TE, class TM>
-void launch_eta_march_ng(Ctx* c, const Bufs<TE, TM>& B) {
+void launch_eta_march_ng(Ctx* c, const Bufs<TE, TM>& B, int g0) {
     const Par& P = c->P;
     const int nTz = (P.nz + 63) / 64, nTy = (P.ny + TY - 1) / TY;
     const int tiles = nTz * nTy;
@@ -169,31 +193,39 @@ void launch_eta_march_ng(Ctx* c, const Bufs<TE, TM>& B) {
                                                            lds);
         resident = std::max(1, per_cu) * std::max(1, ncu);
     }
-    // one wave of resident blocks: X segments long enough to amortise the
-    // extra plane each segment loads at its start
-    int nseg = std::max(1, std::min((resident + tiles - 1) / tiles, P.nx));
+    // each segment loads one extra plane of every field at its start
+    int nseg = pick_nseg(tiles, resident, P.nx, 1.);
     if (c->eta_nseg > 0) nseg = std::min(c->eta_nseg, P.nx);
     const int XS = (P.nx + nseg - 1) / nseg;
     nseg = (P.nx + XS - 1) / XS;
-    for (int g0 = 0; g0 < P.n_eta; g0 += NG)
-        hipLaunchKernelGGL((k_eta_march<PREC, NG, TY, TE, TM>), dim3(tiles * nseg), dim3(64 * TY), lds, c->stream, P,
-                           B, XS, g0);
+    hipLaunchKernelGGL((k_eta_march<PREC, NG, TY, TE, TM>), dim3(tiles * nseg), dim3(64 * TY), lds, c->stream, P, B,
+                       XS, g0);
 }
 template <int PREC, int TY, class TE, class TM>
 void launch_eta_march_ty(Ctx* c, const Bufs<TE, TM>& B) {
-    const int n = c->P.n_eta;
-    // fields per pass: the smallest instantiation that holds them, capped by
-    // eta_ng (passes keep the per-XCD plane working set inside L2)
-    const int cap = c->eta_ng;
-    if (n <= 4) launch_eta_march_ng<PREC, 4, TY>(c, B);
-    else if (n <= 8 || cap <= 8) launch_eta_march_ng<PREC, 8, TY>(c, B);
-    else if (n <= 16 || cap <= 16) launch_eta_march_ng<PREC, 16, TY>(c, B);
-    else launch_eta_march_ng<PREC, 32, TY>(c, B);
+    // Passes of exactly NG fields each: the largest NG <= eta_ng (passes keep
+    // the per-XCD plane working set inside L2) that fits the fields left, then
+    // smaller powers of two for the remainder.
+    const int n = c->P.n_eta, cap = c->eta_ng;
+    for (int g0 = 0; g0 < n;) {
+        const int left = n - g0;
+        if (left >= 32 && cap >= 32) launch_eta_march_ng<PREC, 32, TY>(c, B, g0), g0 += 32;
+        else if (left >= 16 && cap >= 16) launch_eta_march_ng<PREC, 16, TY>(c, B, g0), g0 += 16;
+        else if (left >= 8) launch_eta_march_ng<PREC, 8, TY>(c, B, g0), g0 += 8;
+        else if (left >= 4) launch_eta_march_ng<PREC, 4, TY>(c, B, g0), g0 += 4;
+        else if (left >= 2) launch_eta_march_ng<PREC, 2, TY>(c, B, g0), g0 += 2;
+        else launch_eta_march_ng<PREC, 1, TY>(c, B, g0), g0 += 1;
+    }
 }
 template <int PREC, class TE, class TM>
 bool launch_eta_march(Ctx* c, const Bufs<TE, TM>& B) {
     const int n = c->P.n_eta;
-    if (c->march_disabled || c->dim != 3 || n > 32 || c->P.nz < 32) return false;
+    // the kernel addresses the fields of one pass with 32-bit byte offsets
+    int ng_max = 1;
+    while (ng_max * 2 <= std::min(n, c->eta_ng) && ng_max < 32) ng_max *= 2;
+    if (c->march_disabled || c->dim != 3 || n > 32 || c->P.nz < 32 ||
+        (unsigned long long)c->P.fstride * sizeof(TE) * (unsigned long long)ng_max >= (1ull << 32))
+        return false;
     if constexpr (PREC == PREC_FP64) {
         // binary64 register planes of more than 8 fields limit occupancy to one
         // wave per SIMD; the unblocked kernel is faster there
@@ -225,7 +257,8 @@ int launch_pde_march_ty(Ctx* c, const Bufs<TE, TM>& B) {
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pde_march<PREC, TY, TE, TM>, 64 * TY, lds);
         resident = std::max(1, per_cu) * std::max(1, ncu);
     }
-    int nseg = std::max(1, std::min((resident + tiles - 1) / tiles, P.nx));
+    // the prologue of a segment costs about three planes (4 plane loads, one mu plane)
+    int nseg = pick_nseg(tiles, resident, P.nx, 3.);
     if (c->pde_nseg > 0) nseg = std::min(c->pde_nseg, P.nx);
     while ((long long)tiles * nseg > c->partial_slots && nseg > 1) --nseg;
     const int XS = (P.nx + nseg - 1) / nseg;
@@ -237,8 +270,12 @@ int launch_pde_march_ty(Ctx* c, const Bufs<TE, TM>& B) {
 bool pde_march_applies(const Ctx* c) { return !c->march_disabled && c->dim == 3 && c->P.nz >= 32; }
 template <int PREC, class TE, class TM>
 int launch_pde_march(Ctx* c, const Bufs<TE, TM>& B) {
-    if (c->pde_ty == 2) return launch_pde_march_ty<PREC, 2>(c, B);
-    return launch_pde_march_ty<PREC, 4>(c, B);
+    switch (c->pde_ty) {
+        case 2: return launch_pde_march_ty<PREC, 2>(c, B);
+        case 6: return launch_pde_march_ty<PREC, 6>(c, B);
+        case 8: return launch_pde_march_ty<PREC, 8>(c, B);
+        default: return launch_pde_march_ty<PREC, 4>(c, B);
+    }
 }
 
 template <int DIM, int PREC, class TE, class TM>
@@ -370,7 +407,7 @@ int sync_ctl(Ctx* c, DevCtl* h) {
 }
 
 void* eta_ptr(Ctx* c, int which, int g) {
-    return (char*)c->eta[which] + (size_t)g * c->ncell * c->te;
+    return (char*)c->eta[which] + (size_t)g * c->P.fstride * c->te;
 }
 int mask_index(Ctx* c) { return c->eta_evolves() ? c->cur : 0; }
 
@@ -519,21 +556,49 @@ int nicr_create(const nicr_config* cfg, void** out) {
         return fail(NICR_E_DEVICE, "nicr_create: hipStreamCreate failed");
     }
     const size_t n8 = (size_t)c->ncell * 8, nte = (size_t)c->ncell * c->te;
-    for (int i = 0; i < 2 && rc == NICR_OK; ++i) {
-        rc = dalloc(c, (void**)&c->o[i], n8);
-        if (rc == NICR_OK) rc = dalloc(c, (void**)&c->cr[i], n8);
-        if (rc == NICR_OK) rc = dalloc(c, (void**)&c->p[i], n8);
-    }
-    if (rc == NICR_OK) rc = dalloc(c, &c->mu_o, (size_t)c->ncell * c->tm);
-    if (rc == NICR_OK) rc = dalloc(c, &c->mu_cr, (size_t)c->ncell * c->tm);
+    // State arrays share one slab. Each starts 256-byte aligned and the distance
+    // between consecutive arrays, and between consecutive eta fields, is an odd
+    // multiple of 256 bytes: arrays read at the same cell index then begin on
+    // different HBM channels. With the power-of-two sizes of cubic meshes they
+    // would otherwise all map the same cell to the same channel, which costs
+    // about a sixth of the streaming bandwidth.
+    auto stagger = [](size_t bytes) {
+        size_t b = (bytes + 255) / 256;
+        if (b % 2 == 0) ++b;
+        return b * 256;
+    };
+    P.fstride = (long long)(stagger(nte) / c->te);
     const int nbuf = c->eta_evolves() ? 2 : 1;
-    for (int i = 0; i < nbuf && rc == NICR_OK; ++i) {
-        if (c->has_eta()) {
-            rc = dalloc(c, &c->eta[i], nte * c->cfg.n_eta);
-            if (rc == NICR_OK) rc = dalloc(c, &c->s2[i], nte);
-            if (rc == NICR_OK) rc = dalloc(c, &c->pairs[i], nte);
+    size_t slab_bytes = 6 * stagger(n8) + 2 * stagger((size_t)c->ncell * c->tm);
+    for (int i = 0; i < nbuf; ++i)
+        slab_bytes += (c->has_eta() ? stagger((size_t)P.fstride * c->te * c->cfg.n_eta) + 2 * stagger(nte) : 0)
+                    + stagger(nte);
+    char* slab = nullptr;
+    rc = dalloc(c, (void**)&slab, slab_bytes);
+    if (rc == NICR_OK) {
+        size_t off = 0;
+        auto carve = [&](size_t bytes) {
+            void* q = slab + off;
+            off += stagger(bytes);
+            return q;
+        };
+        for (int i = 0; i < 2; ++i) {
+            c->o[i] = (double*)carve(n8);
+            c->cr[i] = (double*)carve(n8);
+            c->p[i] = (double*)carve(n8);
         }
-        if (rc == NICR_OK) rc = dalloc(c, &c->gb[i], nte);
+        c->mu_o = carve((size_t)c->ncell * c->tm);
+        c->mu_cr = carve((size_t)c->ncell * c->tm);
+        for (int i = 0; i < nbuf; ++i) {
+            if (c->has_eta()) {
+                c->eta[i] = carve((size_t)P.fstride * c->te * c->cfg.n_eta);
+                c->s2[i] = carve(nte);
+                c->pairs[i] = carve(nte);
+            }
+            c->gb[i] = carve(nte);
+        }
+        c->slab = slab;
+        c->slab_bytes = slab_bytes;
     }
     if (nbuf == 1) {
         c->eta[1] = c->eta[0];
@@ -558,20 +623,7 @@ int nicr_create(const nicr_config* cfg, void** out) {
         return fail(NICR_E_DEVICE, m);
     }
     // zero state, control block
-    for (void* p : c->allocs) (void)p;
-    for (int i = 0; i < 2; ++i) {
-        hipMemsetAsync(c->o[i], 0, n8, c->stream);
-        hipMemsetAsync(c->cr[i], 0, n8, c->stream);
-        hipMemsetAsync(c->p[i], 0, n8, c->stream);
-    }
-    for (int i = 0; i < nbuf; ++i) {
-        if (c->has_eta()) {
-            hipMemsetAsync(c->eta[i], 0, nte * c->cfg.n_eta, c->stream);
-            hipMemsetAsync(c->s2[i], 0, nte, c->stream);
-            hipMemsetAsync(c->pairs[i], 0, nte, c->stream);
-        }
-        hipMemsetAsync(c->gb[i], 0, nte, c->stream);
-    }
+    hipMemsetAsync(c->slab, 0, c->slab_bytes, c->stream);
     DevCtl h{};
     h.cur = 0;
     h.pde_key = NO_KEY;

@@ -70,6 +70,7 @@ struct Par {
     int nx, ny, nz, nyz;          // nz == 1 in 2D
     int nchunk, ntiles;           // tiles of TILE cells along the flattened yz plane
     long long ncell;
+    long long fstride;            // elements from one eta field to the next (>= ncell, see nicr_create)
     int n_eta, bc, ck_legacy, bounds_legacy;
     int grains, oxy, has_eta, eta_evolves;
     int enforce_layer, pad0;
@@ -88,7 +89,7 @@ struct Bufs {
     double* p[2];
     TM* mu_o;
     TM* mu_cr;
-    TE* eta[2];     // n_eta * ncell each, field-major
+    TE* eta[2];     // n_eta fields each, field g at g * fstride
     TE* s2[2];
     TE* pairs[2];
     TE* gb[2];      // static mask: gb[0] == gb[1]
@@ -166,6 +167,32 @@ __device__ __forceinline__ DD dd_add(DD a, DD b) {
     const double lo = a.lo + b.lo + err;
     const double hi = s + lo;
     return DD{hi, lo - (hi - s)};
+}
+
+// Raw buffer access (gfx9 buffer resource, dword 3 = 32-bit data format): the
+// address is base + soffset (scalar) + voffset (lane), so one lane offset can
+// serve many fields whose distance is a uniform scalar.
+constexpr int BUF_DWORD3 = 0x00020000;
+template <class T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void buf_store_raw(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void buf_store_raw(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, double v) {
+    using U2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, (int)voff, (int)soff, 0);
+}
+template <class T>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, T v) {
+    buf_store_raw(r, voff, soff, v);
 }
 
 // Neighbour mapping that reproduces ghosts(): X Neumann, Y/Z periodic.

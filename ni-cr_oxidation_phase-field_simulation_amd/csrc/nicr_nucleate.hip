@@ -90,7 +90,7 @@ __device__ int block_stamp(const Par& P, double* o, double* cr, double* p, TE* e
             o[c] = Phys::oxideO;
             cr[c] = Phys::oxideCr;
             p[c] = 1.;
-            for (int g = 0; g < P.n_eta; ++g) eta[(long long)g * P.ncell + c] = (TE)0;
+            for (int g = 0; g < P.n_eta; ++g) eta[(long long)g * P.fstride + c] = (TE)0;
             ++mine;
         }
     }

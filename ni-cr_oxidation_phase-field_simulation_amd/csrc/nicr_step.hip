@@ -28,7 +28,7 @@ __global__ __launch_bounds__(TILE) void k_mask(Par P, Bufs<TE, TM> B, int which)
     if constexpr (PREC == PREC_FP64) {
         double S = 0., pr = 0.;
         for (int g = 0; g < P.n_eta; ++g) {
-            const double v = (double)eta[(long long)g * P.ncell + c];
+            const double v = (double)eta[(long long)g * P.fstride + c];
             const double q = v * v;
             pr += S * q;
             S += q;
@@ -40,7 +40,7 @@ __global__ __launch_bounds__(TILE) void k_mask(Par P, Bufs<TE, TM> B, int which)
     } else {
         float S = 0.f, pr = 0.f;
         for (int g = 0; g < P.n_eta; ++g) {
-            const float v = (float)eta[(long long)g * P.ncell + c];
+            const float v = (float)eta[(long long)g * P.fstride + c];
             const float q = v * v;
             pr += S * q;
             S += q;
@@ -193,62 +193,83 @@ __device__ __forceinline__ DD block_reduce_dd(DD v, DD* sh) {
 // phi update, legacy Ck, finite check and bounds (2D:430-433,435-450,553-591;
 // SL:616-638,771-809). Persistent grid over tiles; one budget partial per block.
 // ---------------------------------------------------------------------------
-// Values of one cell's 7-point stencil (o with the inlet substitution).
-template <class TM>
-struct Cell7 {
-    double o[7], cr[7], p[7], g[7];
-    TM muO[7], muCr[7];
+// mu_O, mu_Cr and the mobilities M_O, M_Cr of one cell (refresh loops 1 and 2,
+// 2D:418-433, SL:604-619). Every face of the trial uses the same per-cell M, and
+// the ghosts are copies, so M(ghost) = M(copy).
+template <class T>
+struct MuMob {
+    T mo, mc, Mo, Mc;
 };
+template <int PREC>
+using MuT = typename std::conditional<PREC == PREC_FP64, double, float>::type;   // mu, M arithmetic type
 
-// Trial values of one cell: mobilities (refresh loop 2, 2D:430-433; ghosts are
-// copies, so M(ghost) = M(copy)), div_flux with the mu-BC inlet face
-// (2D:435-450, SL:621-638), the phi force and update (2D:562-566, SL:780-784).
-template <int DIM, int PREC, class TM>
-__device__ __forceinline__ void trial_core(const Par& P, const Cell7<TM>& s, int x, double S, double& onv,
-                                           double& crv, double& pnv, double& left_flux) {
-    const double oo = s.o[SC], cc = s.cr[SC], pp = s.p[SC];
+template <int PREC, class T>
+__device__ __forceinline__ void mob_vals(const Par& P, double o, double cr, double p, double g, MuMob<T>& r) {
+    if constexpr (PREC == PREC_FP64) {
+        r.Mo = mobO64(P, o, p, g);
+        r.Mc = mobCr64(P, cr, p, g);
+    } else {
+        const float hh = h32((float)p);
+        r.Mo = P.oxy ? mobO32(P, (float)o, hh, (float)g) : 0.f;
+        r.Mc = mobCr32(P, (float)cr, hh, (float)g);
+    }
+}
+
+template <int DIM, int PREC, class T>
+__device__ __forceinline__ MuMob<T> mu_mob(const Par& P, const double* o7, const double* c7, double pc, double g) {
+    MuMob<T> r;
+    if constexpr (PREC == PREC_FP64) {
+        mu64_vals<DIM>(P, o7, c7, pc, r.mo, r.mc);
+    } else {
+        mu32_vals<DIM>(P, o7, c7, pc, r.mo, r.mc);
+    }
+    mob_vals<PREC, T>(P, o7[SC], c7[SC], pc, g, r);
+    return r;
+}
+
+// Trial values of one cell from mu, M at its 7 points, phi at its 7 points and
+// o, cr at the cell: div_flux with the mu-BC inlet face (2D:435-450,
+// SL:621-638), the phi force and update (2D:562-566, SL:780-784).
+// g_c: gb at the cell, m64O: binary64 mu_O at the cell (both for the inlet face
+// only; m64O only in PREC_MIXED).
+template <int DIM, int PREC, class T>
+__device__ __forceinline__ void trial_mm(const Par& P, const MuMob<T>* m, const double* p7, double oo, double cc,
+                                         int x, double S, double g_c, double m64O, double& onv, double& crv,
+                                         double& pnv, double& left_flux) {
+    const double pp = p7[SC];
     left_flux = 0.;
     if constexpr (PREC == PREC_FP64) {
-        const double g_c = s.g[SC];
-        const double moc = mobO64(P, oo, pp, g_c);
-        const double mcc = mobCr64(P, cc, pp, g_c);
-        double mo[7], mc[7];
-#pragma unroll
-        for (int q = 1; q < 7; ++q) {
-            if (DIM == 2 && q >= SZM) break;
-            mo[q] = mobO64(P, s.o[q], s.p[q], s.g[q]);
-            mc[q] = mobCr64(P, s.cr[q], s.p[q], s.g[q]);
-        }
-        const double uo = (double)s.muO[SC], uc = (double)s.muCr[SC];
+        const double moc = m[SC].Mo, mcc = m[SC].Mc;
+        const double uo = m[SC].mo, uc = m[SC].mc;
         double divO = 0.;
         if (P.oxy) {
-            const double right = -.5 * (moc + mo[SXP]) * ((double)s.muO[SXP] - uo) / P.dx;
-            double left = -.5 * (mo[SXM] + moc) * (uo - (double)s.muO[SXM]) / P.dx;
+            const double right = -.5 * (moc + m[SXP].Mo) * (m[SXP].mo - uo) / P.dx;
+            double left = -.5 * (m[SXM].Mo + moc) * (uo - m[SXM].mo) / P.dx;
             if (x == 0 && P.bc == 1) {
                 const double mf = mobO64(P, P.c_ref, pp, g_c);
                 left_flux = 2. * mf * (P.mu_res - uo) / P.dx;
                 left = left_flux;
             }
-            const double top = -.5 * (moc + mo[SYP]) * ((double)s.muO[SYP] - uo) / P.dy;
-            const double bottom = -.5 * (mo[SYM] + moc) * (uo - (double)s.muO[SYM]) / P.dy;
+            const double top = -.5 * (moc + m[SYP].Mo) * (m[SYP].mo - uo) / P.dy;
+            const double bottom = -.5 * (m[SYM].Mo + moc) * (uo - m[SYM].mo) / P.dy;
             double d = (left - right) / P.dx + (bottom - top) / P.dy;
             if (DIM == 3) {
-                const double front = -.5 * (moc + mo[SZP]) * ((double)s.muO[SZP] - uo) / P.dz;
-                const double back = -.5 * (mo[SZM] + moc) * (uo - (double)s.muO[SZM]) / P.dz;
+                const double front = -.5 * (moc + m[SZP].Mo) * (m[SZP].mo - uo) / P.dz;
+                const double back = -.5 * (m[SZM].Mo + moc) * (uo - m[SZM].mo) / P.dz;
                 d = d + (back - front) / P.dz;
             }
             divO = d;
         }
         double divC;
         {
-            const double right = -.5 * (mcc + mc[SXP]) * ((double)s.muCr[SXP] - uc) / P.dx;
-            const double left = -.5 * (mc[SXM] + mcc) * (uc - (double)s.muCr[SXM]) / P.dx;
-            const double top = -.5 * (mcc + mc[SYP]) * ((double)s.muCr[SYP] - uc) / P.dy;
-            const double bottom = -.5 * (mc[SYM] + mcc) * (uc - (double)s.muCr[SYM]) / P.dy;
+            const double right = -.5 * (mcc + m[SXP].Mc) * (m[SXP].mc - uc) / P.dx;
+            const double left = -.5 * (m[SXM].Mc + mcc) * (uc - m[SXM].mc) / P.dx;
+            const double top = -.5 * (mcc + m[SYP].Mc) * (m[SYP].mc - uc) / P.dy;
+            const double bottom = -.5 * (m[SYM].Mc + mcc) * (uc - m[SYM].mc) / P.dy;
             double d = (left - right) / P.dx + (bottom - top) / P.dy;
             if (DIM == 3) {
-                const double front = -.5 * (mcc + mc[SZP]) * ((double)s.muCr[SZP] - uc) / P.dz;
-                const double back = -.5 * (mc[SZM] + mcc) * (uc - (double)s.muCr[SZM]) / P.dz;
+                const double front = -.5 * (mcc + m[SZP].Mc) * (m[SZP].mc - uc) / P.dz;
+                const double back = -.5 * (m[SZM].Mc + mcc) * (uc - m[SZM].mc) / P.dz;
                 d = d + (back - front) / P.dz;
             }
             divC = d;
@@ -256,8 +277,8 @@ __device__ __forceinline__ void trial_core(const Par& P, const Cell7<TM>& s, int
         onv = P.oxy ? oo + P.dt * divO : oo;
         crv = cc + P.dt * divC;
         const double hp = 30. * pp * pp * (pp - 1.) * (pp - 1.);
-        double lp = (s.p[SXP] - 2. * pp + s.p[SXM]) / (P.dx * P.dx) + (s.p[SYP] - 2. * pp + s.p[SYM]) / (P.dy * P.dy);
-        if (DIM == 3) lp = lp + (s.p[SZP] - 2. * pp + s.p[SZM]) / (P.dz * P.dz);
+        double lp = (p7[SXP] - 2. * pp + p7[SXM]) / (P.dx * P.dx) + (p7[SYP] - 2. * pp + p7[SYM]) / (P.dy * P.dy);
+        if (DIM == 3) lp = lp + (p7[SZP] - 2. * pp + p7[SZM]) / (P.dz * P.dz);
         double fp = Phys::W * (4. * pp * pp * pp - 6. * pp * pp + 2. * pp)
                   + hp * (foxide64(oo, cc) - fmetal64(oo, cc)) - 2. * Phys::kp * lp;
         if (P.has_eta) fp += 0.5 * hp * (P.etaW * (S - (1. - h64(pp))) + P.etaA * S);
@@ -265,29 +286,19 @@ __device__ __forceinline__ void trial_core(const Par& P, const Cell7<TM>& s, int
     } else {
 #pragma clang fp contract(fast)
         const float hc = h32((float)pp);
-        const float g_c = (float)s.g[SC];
-        const float moc = P.oxy ? mobO32(P, (float)oo, hc, g_c) : 0.f;
-        const float mcc = mobCr32(P, (float)cc, hc, g_c);
-        const float uo = (float)s.muO[SC], uc = (float)s.muCr[SC];
+        const float moc = m[SC].Mo, mcc = m[SC].Mc;
+        const float uo = m[SC].mo, uc = m[SC].mc;
         float sumO = 0.f, sumC = 0.f;   // sum over faces of M_face * (mu_nb - mu_c) / h^2
         auto face = [&](int q, float inv2, bool with_o) {
-            const float hq = h32((float)s.p[q]);
-            const float gq = (float)s.g[q];
-            if (P.oxy && with_o) {
-                const float mq = mobO32(P, (float)s.o[q], hq, gq);
-                sumO += 0.5f * (moc + mq) * ((float)s.muO[q] - uo) * inv2;
-            }
-            const float mq2 = mobCr32(P, (float)s.cr[q], hq, gq);
-            sumC += 0.5f * (mcc + mq2) * ((float)s.muCr[q] - uc) * inv2;
+            if (P.oxy && with_o) sumO += 0.5f * (moc + m[q].Mo) * (m[q].mo - uo) * inv2;
+            sumC += 0.5f * (mcc + m[q].Mc) * (m[q].mc - uc) * inv2;
         };
         face(SXP, P.fdx2i, true);
         const bool inlet = x == 0 && P.bc == 1;
         face(SXM, P.fdx2i, !inlet);
         if (inlet && P.oxy) {
             // left face replaced by the binary64 inlet flux (2D:435-442,446)
-            double m64O, m64C;
-            mu64_vals<DIM>(P, s.o, s.cr, pp, m64O, m64C);
-            const double mf = mobO64(P, P.c_ref, pp, s.g[SC]);
+            const double mf = mobO64(P, P.c_ref, pp, g_c);
             left_flux = 2. * mf * (P.mu_res - m64O) / P.dx;
             sumO += (float)(left_flux / P.dx);
         }
@@ -301,8 +312,8 @@ __device__ __forceinline__ void trial_core(const Par& P, const Cell7<TM>& s, int
         crv = cc + P.dt * (double)sumC;
         const float fpp = (float)pp;
         const float hp = 30.f * fpp * fpp * (fpp - 1.f) * (fpp - 1.f);
-        const float lp = (float)(s.p[SXP] - 2. * pp + s.p[SXM]) * P.fdx2i + (float)(s.p[SYP] - 2. * pp + s.p[SYM]) * P.fdy2i
-                       + (DIM == 3 ? (float)(s.p[SZP] - 2. * pp + s.p[SZM]) * P.fdz2i : 0.f);
+        const float lp = (float)(p7[SXP] - 2. * pp + p7[SXM]) * P.fdx2i + (float)(p7[SYP] - 2. * pp + p7[SYM]) * P.fdy2i
+                       + (DIM == 3 ? (float)(p7[SZP] - 2. * pp + p7[SZM]) * P.fdz2i : 0.f);
         const float fo = (float)oo, fcr = (float)cc;
         // foxide - fmetal, expanded (2D:174-181)
         const float dfx = 6000000.f * fo * fo + (7000000.f - 583264.303945031f) * fcr * fcr - 17000000.f * fo * fcr
@@ -396,20 +407,30 @@ __global__ __launch_bounds__(TILE) void k_trial(Par P, Bufs<TE, TM> B) {
         if (!tile_cell(P, t, x, y, z)) continue;
         const Nbr n = neighbours<DIM>(P, x, y, z);
         const long long c = n.c;
-        Cell7<TM> s;
-        gather_oc<DIM>(P, o, cr, n, enf, s.o, s.cr);
+        using T = MuT<PREC>;
+        double o7[7], c7[7], p7[7];
+        MuMob<T> m[7];
+        gather_oc<DIM>(P, o, cr, n, enf, o7, c7);
         const long long idx[7] = {n.c, n.xm, n.xp, n.ym, n.yp, n.zm, n.zp};
 #pragma unroll
         for (int q = 0; q < 7; ++q) {
-            s.p[q] = p[idx[q]];
-            s.g[q] = (double)gb[idx[q]];
-            s.muO[q] = B.mu_o[idx[q]];
-            s.muCr[q] = B.mu_cr[idx[q]];
+            if (DIM == 2 && q >= SZM) break;
+            p7[q] = p[idx[q]];
+            m[q].mo = (T)B.mu_o[idx[q]];
+            m[q].mc = (T)B.mu_cr[idx[q]];
+            mob_vals<PREC, T>(P, o7[q], c7[q], p7[q], (double)gb[idx[q]], m[q]);
+        }
+        double m64O = 0.;
+        if constexpr (PREC != PREC_FP64) {
+            if (x == 0 && P.bc == 1 && P.oxy) {
+                double m64C;
+                mu64_vals<DIM>(P, o7, c7, p7[SC], m64O, m64C);
+            }
         }
         const double S = P.has_eta ? (double)s2[c] : 0.;
         double onv, crv, pnv, left_flux;
-        trial_core<DIM, PREC, TM>(P, s, x, S, onv, crv, pnv, left_flux);
-        finish_cell(P, c, x, o[c], s.o[SC], s.cr[SC], s.p[SC], onv, crv, pnv, left_flux, acc, B.o[nxt], B.cr[nxt],
+        trial_mm<DIM, PREC, T>(P, m, p7, o7[SC], c7[SC], x, S, (double)gb[c], m64O, onv, crv, pnv, left_flux);
+        finish_cell(P, c, x, o[c], o7[SC], c7[SC], p7[SC], onv, crv, pnv, left_flux, acc, B.o[nxt], B.cr[nxt],
                     B.p[nxt]);
     }
     write_partials(acc, B.partials, ctl);
@@ -418,42 +439,52 @@ __global__ __launch_bounds__(TILE) void k_trial(Par P, Bufs<TE, TM> B) {
 // ---------------------------------------------------------------------------
 // k_pde_march (3D): k_mu and k_trial fused, so mu never makes an HBM round trip
 // and every o, cr, phi, gb value is fetched from HBM once. A block owns a 64 (z)
-// x TY (y) column tile and marches along X over XS planes:
-//   LDS ring of 4 planes of o, cr (two-cell Y/Z halo) and phi, gb (one-cell halo),
-//   LDS ring of 2 planes of mu_O, mu_Cr (one-cell halo),
-//   own-cell values of plane x-1 in registers.
-// Iteration x: load plane x+2 | sync | mu(x+1) over tile+halo | sync | trial(x).
-// The arithmetic is trial_core / mu*_vals, shared with k_mu + k_trial, so the
-// binary64 results are bitwise equal to theirs.
+// x TY (y) column tile and marches along X over XS planes. LDS holds
+//   o, cr      ring of 3 planes with a two-cell Y/Z halo,
+//   phi        ring of 3 planes, gb ring of 2 planes, one-cell halo,
+//   mu, M      2 planes with a one-cell halo, evaluated once per cell,
+// and the own cell's values of plane x-1 sit in registers. Iteration x:
+//   A  global loads of o, cr (plane x+3), phi, gb (plane x+2) into registers
+//   B  mu, M of plane x+1 over the tile and its halo
+//   -- barrier --
+//   C  trial of plane x (needs only mu, M, phi of the neighbours)
+//   D  the registers of A into the ring slots that B and C no longer read
+//   -- barrier --
+// so the HBM latency of A is hidden behind B and C. The ring offsets of each
+// thread are fixed for the block and computed once. The binary64 arithmetic is
+// that of k_mu + k_trial (mu64_vals, mob_vals, trial_mm), so both give bitwise
+// equal results.
 // ---------------------------------------------------------------------------
 template <int PREC, int TY>
 struct PdeLds {
+    static constexpr int NTH = 64 * TY;
     static constexpr int W2 = 68, H2 = TY + 4, W1 = 66, H1 = TY + 2;
     static constexpr int P2 = H2 * W2, P1 = H1 * W1;
-    using TMU = typename std::conditional<PREC == PREC_FP64, double, float>::type;
-    using TG = TMU;
+    static constexpr int R2 = (P2 + NTH - 1) / NTH, R1 = (P1 + NTH - 1) / NTH;   // ring entries per thread
+    using T = MuT<PREC>;
     static constexpr size_t bytes() {
-        return size_t(4) * P2 * 8 * 2 + size_t(4) * P1 * 8 + size_t(4) * P1 * sizeof(TG) + size_t(2) * P1 * sizeof(TMU) * 2;
+        return size_t(2) * P1 * sizeof(MuMob<T>) + size_t(3) * P2 * 8 * 2 + size_t(3) * P1 * 8
+             + size_t(2) * P1 * sizeof(T);
     }
 };
 
 template <int PREC, int TY, class TE, class TM>
 __global__ __launch_bounds__(64 * TY) void k_pde_march(Par P, Bufs<TE, TM> B, int XS) {
     using L = PdeLds<PREC, TY>;
-    using TMU = typename L::TMU;
-    using TG = typename L::TG;
-    extern __shared__ unsigned char lds_raw[];
-    double* so = reinterpret_cast<double*>(lds_raw);             // [4][P2]
-    double* sc = so + 4 * L::P2;                                  // [4][P2]
-    double* sp = sc + 4 * L::P2;                                  // [4][P1]
-    TG* sg = reinterpret_cast<TG*>(sp + 4 * L::P1);               // [4][P1]
-    TMU* smo = reinterpret_cast<TMU*>(sg + 4 * L::P1);            // [2][P1]
-    TMU* smc = smo + 2 * L::P1;                                   // [2][P1]
+    using T = typename L::T;
+    using MM = MuMob<T>;
+    constexpr int NTH = L::NTH, P1 = L::P1, P2 = L::P2, W1 = L::W1, W2 = L::W2, R1 = L::R1, R2 = L::R2;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    MM* smm = reinterpret_cast<MM*>(lds_raw);                  // [2][P1] mu, M
+    double* so = reinterpret_cast<double*>(smm + 2 * P1);      // [3][P2] o
+    double* sc = so + 3 * P2;                                  // [3][P2] cr
+    double* sp = sc + 3 * P2;                                  // [3][P1] phi
+    T* sg = reinterpret_cast<T*>(sp + 3 * P1);                 // [2][P1] gb
     DevCtl* ctl = B.ctl;
     if (ctl->failed) return;
     const int cur = ctl->cur, nxt = cur ^ 1;
-    const int nth = 64 * TY;
-    const int tz = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int tid = threadIdx.x;
+    const int tz = tid & 63, ty = tid >> 6;
     const int nTz = (P.nz + 63) >> 6, nTy = (P.ny + TY - 1) / TY, nSeg = (P.nx + XS - 1) / XS;
     const int b = xcd_remap(blockIdx.x, nTz * nTy * nSeg);
     const int seg = b / (nTz * nTy);
@@ -475,166 +506,204 @@ __global__ __launch_bounds__(64 * TY) void k_pde_march(Par P, Bufs<TE, TM> B, in
     auto wrapz = [&](int v) { return ((v % P.nz) + P.nz) % P.nz; };
     auto clampx = [&](int v) { return v < 0 ? 0 : (v >= P.nx ? P.nx - 1 : v); };
 
-    // plane X (clamped: the X ghosts are copies) into ring slot X & 3
-    auto load_plane = [&](int X) {
-        const int xc = clampx(X), slot = X & 3;
-        const long long xo = (long long)xc * plane;
+    // Ring entries of this thread: entry r is cell tid + r*NTH of the halo-2
+    // (o, cr) or halo-1 (phi, gb, mu, M) plane tile. Global in-plane offsets with
+    // the Y/Z wrap applied, and for halo-1 entries the index into halo-2 planes.
+    int off2[R2], off1[R1], q2of[R1];
+#pragma unroll
+    for (int r = 0; r < R2; ++r) {
+        const int q = tid + r * NTH, ly = q / W2, lz = q - ly * W2;
+        off2[r] = q < P2 ? wrapy(y0 - 2 + ly) * P.nz + wrapz(z0 - 2 + lz) : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+        const int q = tid + r * NTH, ly = q / W1, lz = q - ly * W1;
+        off1[r] = q < P1 ? wrapy(y0 - 1 + ly) * P.nz + wrapz(z0 - 1 + lz) : 0;
+        q2of[r] = (ly + 1) * W2 + (lz + 1);
+    }
+    auto in2 = [&](int r) { return (r + 1) * NTH <= P2 || tid + r * NTH < P2; };
+    auto in1 = [&](int r) { return (r + 1) * NTH <= P1 || tid + r * NTH < P1; };
+
+    // staging registers: o, cr of one plane, phi, gb of one plane
+    double ro[R2], rc[R2], rp[R1];
+    T rg[R1];
+    auto fetch_oc = [&](int X) {   // X clamped: the X ghosts are copies
+        const int xc = clampx(X);
+        const double* bo = o + (long long)xc * plane;
+        const double* bc = cr + (long long)xc * plane;
         const bool sub = enf && xc == 0;
-        for (int q = threadIdx.x; q < L::P2; q += nth) {
-            const int ly = q / L::W2, lz = q - ly * L::W2;
-            const long long g = xo + (long long)wrapy(y0 - 2 + ly) * P.nz + wrapz(z0 - 2 + lz);
-            so[slot * L::P2 + q] = sub ? Phys::layerO : o[g];
-            sc[slot * L::P2 + q] = cr[g];
-        }
-        for (int q = threadIdx.x; q < L::P1; q += nth) {
-            const int ly = q / L::W1, lz = q - ly * L::W1;
-            const long long g = xo + (long long)wrapy(y0 - 1 + ly) * P.nz + wrapz(z0 - 1 + lz);
-            sp[slot * L::P1 + q] = p[g];
-            sg[slot * L::P1 + q] = (TG)gb[g];
-        }
-    };
-    // 7-point o, cr around halo-1 cell (ly1, lz1) of plane X from the rings
-    // (with_xm false: the x-1 entries are left for the caller to fill)
-    auto oc7 = [&](int X, int ly1, int lz1, double* o7, double* c7, bool with_xm) {
-        const int s = X & 3, sm = (X - 1) & 3, sp_ = (X + 1) & 3;
-        const int q = (ly1 + 1) * L::W2 + (lz1 + 1);
-        o7[SC] = so[s * L::P2 + q];
-        c7[SC] = sc[s * L::P2 + q];
-        if (with_xm) {
-            o7[SXM] = so[sm * L::P2 + q];
-            c7[SXM] = sc[sm * L::P2 + q];
-        }
-        o7[SXP] = so[sp_ * L::P2 + q];
-        c7[SXP] = sc[sp_ * L::P2 + q];
-        o7[SYM] = so[s * L::P2 + q - L::W2];
-        c7[SYM] = sc[s * L::P2 + q - L::W2];
-        o7[SYP] = so[s * L::P2 + q + L::W2];
-        c7[SYP] = sc[s * L::P2 + q + L::W2];
-        o7[SZM] = so[s * L::P2 + q - 1];
-        c7[SZM] = sc[s * L::P2 + q - 1];
-        o7[SZP] = so[s * L::P2 + q + 1];
-        c7[SZP] = sc[s * L::P2 + q + 1];
-    };
-    // mu of plane X over the tile and its one-cell halo, into mu ring slot X & 1
-    auto mu_plane = [&](int X) {
-        const int ms = X & 1;
-        for (int q = threadIdx.x; q < L::P1; q += nth) {
-            const int ly1 = q / L::W1, lz1 = q - ly1 * L::W1;
-            double o7[7], c7[7];
-            oc7(X, ly1, lz1, o7, c7, true);
-            const double pc = sp[(X & 3) * L::P1 + q];
-            if constexpr (PREC == PREC_FP64) {
-                double a, bb;
-                mu64_vals<3>(P, o7, c7, pc, a, bb);
-                smo[ms * L::P1 + q] = a;
-                smc[ms * L::P1 + q] = bb;
-            } else {
-                float a, bb;
-                mu32_vals<3>(P, o7, c7, pc, a, bb);
-                smo[ms * L::P1 + q] = a;
-                smc[ms * L::P1 + q] = bb;
+#pragma unroll
+        for (int r = 0; r < R2; ++r)
+            if (in2(r)) {
+                ro[r] = sub ? Phys::layerO : bo[off2[r]];
+                rc[r] = bc[off2[r]];
             }
+    };
+    auto store_oc = [&](int slot) {
+#pragma unroll
+        for (int r = 0; r < R2; ++r)
+            if (in2(r)) {
+                so[slot * P2 + tid + r * NTH] = ro[r];
+                sc[slot * P2 + tid + r * NTH] = rc[r];
+            }
+    };
+    auto fetch_pg = [&](int X) {
+        const int xc = clampx(X);
+        const double* bp = p + (long long)xc * plane;
+        const TE* bg = gb + (long long)xc * plane;
+#pragma unroll
+        for (int r = 0; r < R1; ++r)
+            if (in1(r)) {
+                rp[r] = bp[off1[r]];
+                rg[r] = (T)bg[off1[r]];
+            }
+    };
+    auto store_pg = [&](int ps, int gs) {
+#pragma unroll
+        for (int r = 0; r < R1; ++r)
+            if (in1(r)) {
+                sp[ps * P1 + tid + r * NTH] = rp[r];
+                sg[gs * P1 + tid + r * NTH] = rg[r];
+            }
+    };
+    // mu, M of one plane over the tile and its one-cell halo. Ring slots: o, cr
+    // of the plane (sx) and of its X neighbours (sm, sq), phi (ps), gb (gs);
+    // output mu, M slot ms.
+    auto mm_plane = [&](int sm, int sx, int sq, int ps, int gs, int ms) {
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            if (!in1(r)) continue;
+            const int q = tid + r * NTH, q2 = q2of[r];
+            double o7[7], c7[7];
+            o7[SC] = so[sx * P2 + q2];
+            c7[SC] = sc[sx * P2 + q2];
+            o7[SXM] = so[sm * P2 + q2];
+            c7[SXM] = sc[sm * P2 + q2];
+            o7[SXP] = so[sq * P2 + q2];
+            c7[SXP] = sc[sq * P2 + q2];
+            o7[SYM] = so[sx * P2 + q2 - W2];
+            c7[SYM] = sc[sx * P2 + q2 - W2];
+            o7[SYP] = so[sx * P2 + q2 + W2];
+            c7[SYP] = sc[sx * P2 + q2 + W2];
+            o7[SZM] = so[sx * P2 + q2 - 1];
+            c7[SZM] = sc[sx * P2 + q2 - 1];
+            o7[SZP] = so[sx * P2 + q2 + 1];
+            c7[SZP] = sc[sx * P2 + q2 + 1];
+            smm[ms * P1 + q] = mu_mob<3, PREC, T>(P, o7, c7, sp[ps * P1 + q], (double)sg[gs * P1 + q]);
         }
     };
 
-    // prologue: planes x0-1, x0, x0+1; mu(x0); own-cell values of plane x0-1
-    load_plane(x0 - 1);
-    load_plane(x0);
-    load_plane(x0 + 1);
+    // Prologue. o, cr of plane x0-1+k sit in ring slot k % 3; phi of plane x0+k
+    // in slot k % 3; gb and mu, M of plane x0+k in slot k & 1.
+    fetch_oc(x0 - 1);
+    store_oc(0);
+    fetch_oc(x0);
+    store_oc(1);
+    fetch_oc(x0 + 1);
+    store_oc(2);
+    fetch_pg(x0);
+    store_pg(0, 0);
+    fetch_pg(x0 + 1);
+    store_pg(1, 1);
     __syncthreads();
-    mu_plane(x0);
-    __syncthreads();
-    const int own1 = (ty + 1) * L::W1 + (tz + 1);
-    const int own2 = (ty + 2) * L::W2 + (tz + 2);
-    double o_prev, c_prev, p_prev;
-    TG g_prev;
-    TMU mo_prev, mc_prev;
+    fetch_oc(x0 + 2);
+    mm_plane(0, 1, 2, 0, 0, 0);
+    // own cell of plane x0-1 (for x0 == 0 the X ghost, a copy of plane 0, mu
+    // included: 2D:427) straight from global memory
+    MM m_prev;
+    double p_prev;
     {
-        const int s = (x0 - 1) & 3;
-        o_prev = so[s * L::P2 + own2];
-        c_prev = sc[s * L::P2 + own2];
-        p_prev = sp[s * L::P1 + own1];
-        g_prev = sg[s * L::P1 + own1];
-        if (x0 == 0) {
-            // the x-1 ghost of the first plane is a copy of it, mu included (2D:427)
-            mo_prev = smo[(x0 & 1) * L::P1 + own1];
-            mc_prev = smc[(x0 & 1) * L::P1 + own1];
-        } else {
-            // mu(x0-1) at the own cell: its x-1 neighbour is plane x0-2 (global)
-            double o7[7], c7[7];
-            oc7(x0 - 1, ty + 1, tz + 1, o7, c7, false);
-            const int xmm = clampx(x0 - 2);
-            const long long gm = (long long)xmm * plane + (long long)wrapy(y) * P.nz + wrapz(z);
-            o7[SXM] = (enf && xmm == 0) ? Phys::layerO : o[gm];
-            c7[SXM] = cr[gm];
-            if constexpr (PREC == PREC_FP64) {
-                double a, bb;
-                mu64_vals<3>(P, o7, c7, p_prev, a, bb);
-                mo_prev = a;
-                mc_prev = bb;
-            } else {
-                float a, bb;
-                mu32_vals<3>(P, o7, c7, p_prev, a, bb);
-                mo_prev = a;
-                mc_prev = bb;
-            }
-        }
+        const int yq = y < P.ny ? y : y % P.ny, zq = z < P.nz ? z : z % P.nz;
+        const Nbr n = neighbours<3>(P, x0 > 0 ? x0 - 1 : 0, yq, zq);
+        double o7[7], c7[7];
+        gather_oc<3>(P, o, cr, n, enf, o7, c7);
+        p_prev = p[n.c];
+        m_prev = mu_mob<3, PREC, T>(P, o7, c7, p_prev, (double)gb[n.c]);
     }
+    __syncthreads();
+    store_oc(0);   // plane x0+2 over plane x0-1
+    __syncthreads();
+    const int own1 = (ty + 1) * W1 + (tz + 1);
+    const int own2 = (ty + 2) * W2 + (tz + 2);
+    const long long yz = (long long)(y < P.ny ? y : 0) * P.nz + (z < P.nz ? z : 0);
+    int so_x = 1, so_x1 = 2, so_x2 = 0;   // o, cr slots of planes x, x+1, x+2
+    int sp_x = 0, sp_x1 = 1, sp_x2 = 2;   // phi slots of planes x, x+1 and of the next plane loaded
     StepAcc acc;
     for (int x = x0; x < x1; ++x) {
-        load_plane(x + 2);
-        __syncthreads();
-        mu_plane(x + 1);
-        __syncthreads();
-        const int s = x & 3, s1 = (x + 1) & 3, ms = x & 1, ms1 = (x + 1) & 1;
-        Cell7<TMU> st;
-        oc7(x, ty + 1, tz + 1, st.o, st.cr, false);
-        st.o[SXM] = o_prev;
-        st.cr[SXM] = c_prev;
-        st.p[SC] = sp[s * L::P1 + own1];
-        st.p[SXM] = p_prev;
-        st.p[SXP] = sp[s1 * L::P1 + own1];
-        st.p[SYM] = sp[s * L::P1 + own1 - L::W1];
-        st.p[SYP] = sp[s * L::P1 + own1 + L::W1];
-        st.p[SZM] = sp[s * L::P1 + own1 - 1];
-        st.p[SZP] = sp[s * L::P1 + own1 + 1];
-        st.g[SC] = (double)sg[s * L::P1 + own1];
-        st.g[SXM] = (double)g_prev;
-        st.g[SXP] = (double)sg[s1 * L::P1 + own1];
-        st.g[SYM] = (double)sg[s * L::P1 + own1 - L::W1];
-        st.g[SYP] = (double)sg[s * L::P1 + own1 + L::W1];
-        st.g[SZM] = (double)sg[s * L::P1 + own1 - 1];
-        st.g[SZP] = (double)sg[s * L::P1 + own1 + 1];
-        st.muO[SC] = smo[ms * L::P1 + own1];
-        st.muCr[SC] = smc[ms * L::P1 + own1];
-        st.muO[SXM] = mo_prev;
-        st.muCr[SXM] = mc_prev;
-        // the x+1 ghost of the last plane is a copy of it, mu included
-        st.muO[SXP] = x + 1 == P.nx ? st.muO[SC] : smo[ms1 * L::P1 + own1];
-        st.muCr[SXP] = x + 1 == P.nx ? st.muCr[SC] : smc[ms1 * L::P1 + own1];
-        st.muO[SYM] = smo[ms * L::P1 + own1 - L::W1];
-        st.muCr[SYM] = smc[ms * L::P1 + own1 - L::W1];
-        st.muO[SYP] = smo[ms * L::P1 + own1 + L::W1];
-        st.muCr[SYP] = smc[ms * L::P1 + own1 + L::W1];
-        st.muO[SZM] = smo[ms * L::P1 + own1 - 1];
-        st.muCr[SZM] = smc[ms * L::P1 + own1 - 1];
-        st.muO[SZP] = smo[ms * L::P1 + own1 + 1];
-        st.muCr[SZP] = smc[ms * L::P1 + own1 + 1];
-        if (active) {
-            const long long c = (long long)x * plane + (long long)y * P.nz + z;
-            const double S = P.has_eta ? (double)s2[c] : 0.;
-            const double o_stored = x == 0 ? o[c] : st.o[SC];
-            double onv, crv, pnv, left_flux;
-            trial_core<3, PREC, TMU>(P, st, x, S, onv, crv, pnv, left_flux);
-            finish_cell(P, c, x, o_stored, st.o[SC], st.cr[SC], st.p[SC], onv, crv, pnv, left_flux, acc, B.o[nxt],
-                        B.cr[nxt], B.p[nxt]);
+        const int m0 = (x - x0) & 1, m1 = m0 ^ 1;   // gb, mu/M slots of planes x (and x+2), x+1
+        // A: planes x+3 (o, cr) and x+2 (phi, gb) into registers
+        fetch_oc(x + 3);
+        fetch_pg(x + 2);
+        const long long c = (long long)x * plane + yz;
+        const double S = active && P.has_eta ? (double)s2[c] : 0.;
+        // B: mu, M of plane x+1; o, cr of the own cell of plane x
+        mm_plane(so_x, so_x1, so_x2, sp_x1, m1, m1);
+        const double oo = so[so_x * P2 + own2], cc = sc[so_x * P2 + own2];
+        double m64O = 0.;
+        if constexpr (PREC != PREC_FP64) {
+            if (x == 0 && P.bc == 1 && P.oxy) {
+                // binary64 mu_O for the inlet face; the x-1 ghost of plane 0 is a copy
+                double o7[7], c7[7], m64C;
+                o7[SC] = o7[SXM] = oo;
+                c7[SC] = c7[SXM] = cc;
+                o7[SXP] = so[so_x1 * P2 + own2];
+                c7[SXP] = sc[so_x1 * P2 + own2];
+                o7[SYM] = so[so_x * P2 + own2 - W2];
+                c7[SYM] = sc[so_x * P2 + own2 - W2];
+                o7[SYP] = so[so_x * P2 + own2 + W2];
+                c7[SYP] = sc[so_x * P2 + own2 + W2];
+                o7[SZM] = so[so_x * P2 + own2 - 1];
+                c7[SZM] = sc[so_x * P2 + own2 - 1];
+                o7[SZP] = so[so_x * P2 + own2 + 1];
+                c7[SZP] = sc[so_x * P2 + own2 + 1];
+                mu64_vals<3>(P, o7, c7, sp[sp_x * P1 + own1], m64O, m64C);
+            }
         }
-        o_prev = st.o[SC];
-        c_prev = st.cr[SC];
-        p_prev = st.p[SC];
-        g_prev = sg[s * L::P1 + own1];
-        mo_prev = st.muO[SC];
-        mc_prev = st.muCr[SC];
+        __syncthreads();
+        // C: trial of the own cell of plane x
+        MM m[7];
+        double p7[7];
+        m[SC] = smm[m0 * P1 + own1];
+        m[SYM] = smm[m0 * P1 + own1 - W1];
+        m[SYP] = smm[m0 * P1 + own1 + W1];
+        m[SZM] = smm[m0 * P1 + own1 - 1];
+        m[SZP] = smm[m0 * P1 + own1 + 1];
+        m[SXP] = smm[m1 * P1 + own1];
+        if (x + 1 == P.nx) {
+            // the x+1 ghost of the last plane is a copy of it, mu included
+            m[SXP].mo = m[SC].mo;
+            m[SXP].mc = m[SC].mc;
+        }
+        m[SXM] = m_prev;
+        p7[SC] = sp[sp_x * P1 + own1];
+        p7[SYM] = sp[sp_x * P1 + own1 - W1];
+        p7[SYP] = sp[sp_x * P1 + own1 + W1];
+        p7[SZM] = sp[sp_x * P1 + own1 - 1];
+        p7[SZP] = sp[sp_x * P1 + own1 + 1];
+        p7[SXP] = sp[sp_x1 * P1 + own1];
+        p7[SXM] = p_prev;
+        if (active) {
+            const double g_c = x == 0 ? (double)gb[c] : 0.;
+            const double o_stored = x == 0 ? o[c] : oo;
+            double onv, crv, pnv, left_flux;
+            trial_mm<3, PREC, T>(P, m, p7, oo, cc, x, S, g_c, m64O, onv, crv, pnv, left_flux);
+            finish_cell(P, c, x, o_stored, oo, cc, p7[SC], onv, crv, pnv, left_flux, acc, B.o[nxt], B.cr[nxt],
+                        B.p[nxt]);
+        }
+        m_prev = m[SC];
+        p_prev = p7[SC];
+        // D: staged planes into the slots of planes x (o, cr; gb) and x-1 (phi)
+        store_oc(so_x);
+        store_pg(sp_x2, m0);
+        __syncthreads();
+        const int t = so_x;
+        so_x = so_x1;
+        so_x1 = so_x2;
+        so_x2 = t;
+        const int u = sp_x;
+        sp_x = sp_x1;
+        sp_x1 = sp_x2;
+        sp_x2 = u;
     }
     write_partials(acc, B.partials, ctl);
 }
@@ -664,14 +733,14 @@ __global__ __launch_bounds__(TILE) void k_eta(Par P, Bufs<TE, TM> B) {
             double S2 = 0., pr = 0.;
             const double dtl = P.dt * P.etaL;
             for (int g = 0; g < P.n_eta; ++g) {
-                const long long off = (long long)g * P.ncell;
+                const long long off = (long long)g * P.fstride;
                 const TE* e = ein + off;
                 const double ec = (double)e[c];
                 const double lp = lap64<DIM>((const double*)e, n, ec, P, (double)e[n.xm], (double)e[n.xp]);
                 const double dF = ec * (P.etaW * (S - (1. - h64(pp)) + 2. * (S - ec * ec)) + P.etaA * h64(pp));
                 const double v = ec - dtl * (dF - P.etaK * lp);
                 if (!isfinite(v) || v < -1.e-12 || v > 1. + 1.e-12) {
-                    const unsigned long long key = (unsigned long long)off + (unsigned long long)c;
+                    const unsigned long long key = (unsigned long long)g * (unsigned long long)P.ncell + (unsigned long long)c;
                     bad = bad < key ? bad : key;
                 }
                 eout[off + c] = (TE)v;
@@ -693,7 +762,7 @@ __global__ __launch_bounds__(TILE) void k_eta(Par P, Bufs<TE, TM> B) {
             const float K = (float)P.etaK;
             float S2 = 0.f, pr = 0.f;
             for (int g = 0; g < P.n_eta; ++g) {
-                const long long off = (long long)g * P.ncell;
+                const long long off = (long long)g * P.fstride;
                 const TE* e = ein + off;
                 const float ec = (float)e[c];
                 float lp = ((float)e[n.xp] - 2.f * ec + (float)e[n.xm]) * P.fdx2i
@@ -702,7 +771,7 @@ __global__ __launch_bounds__(TILE) void k_eta(Par P, Bufs<TE, TM> B) {
                 const float dF = ec * (base - 2.f * (float)P.etaW * ec * ec + A);
                 const float v = ec - dtl * (dF - K * lp);
                 if (!isfinite(v) || v < -1.e-12f || v > 1.f + 1.e-12f) {
-                    const unsigned long long key = (unsigned long long)off + (unsigned long long)c;
+                    const unsigned long long key = (unsigned long long)g * (unsigned long long)P.ncell + (unsigned long long)c;
                     bad = bad < key ? bad : key;
                 }
                 eout[off + c] = (TE)v;
@@ -756,8 +825,8 @@ __global__ __launch_bounds__(64 * TY) void k_eta_march(Par P, Bufs<TE, TM> B, in
     // cell, so the LDS tile row/column after the last active one is exactly
     // the Y/Z periodic neighbour; they compute and store nothing.
     const long long rc = (long long)(y % P.ny) * P.nz + (z % P.nz);
-    const TE* ein = B.eta[cur_i] + (long long)g0 * P.ncell;
-    TE* eout = B.eta[nxt_i] + (long long)g0 * P.ncell;
+    const TE* ein = B.eta[cur_i] + (long long)g0 * P.fstride;
+    TE* eout = B.eta[nxt_i] + (long long)g0 * P.fstride;
     const int ng = min(NG, P.n_eta - g0);
     const bool first_pass = g0 == 0, last_pass = g0 + NG >= P.n_eta;
     const long long key0 = (long long)g0 * P.ncell;
@@ -768,8 +837,8 @@ __global__ __launch_bounds__(64 * TY) void k_eta_march(Par P, Bufs<TE, TM> B, in
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             if (g < ng) {
-                prev[g] = ein[(long long)g * P.ncell + xp * plane + rc];
-                cv[g] = ein[(long long)g * P.ncell + (long long)x0 * plane + rc];
+                prev[g] = ein[(long long)g * P.fstride + xp * plane + rc];
+                cv[g] = ein[(long long)g * P.fstride + (long long)x0 * plane + rc];
             } else {
                 prev[g] = T(0);
                 cv[g] = T(0);
@@ -785,7 +854,7 @@ __global__ __launch_bounds__(64 * TY) void k_eta_march(Par P, Bufs<TE, TM> B, in
         // issue the next plane's loads early (register prefetch)
 #pragma unroll
         for (int g = 0; g < NG; ++g)
-            nx_[g] = g < ng ? ein[(long long)g * P.ncell + xn + rc] : T(0);
+            nx_[g] = g < ng ? ein[(long long)g * P.fstride + xn + rc] : T(0);
         // interior of plane x into LDS
 #pragma unroll
         for (int g = 0; g < NG; ++g)
@@ -823,7 +892,7 @@ __global__ __launch_bounds__(64 * TY) void k_eta_march(Par P, Bufs<TE, TM> B, in
 #pragma unroll
             for (int u = 0; u < GC; ++u) {
                 const int g = gk * GC + u;
-                hv[u] = g < ng ? ein[(long long)g * P.ncell + hc] : T(0);
+                hv[u] = g < ng ? ein[(long long)g * P.fstride + hc] : T(0);
             }
 #pragma unroll
             for (int u = 0; u < GC; ++u) {
@@ -860,7 +929,7 @@ __global__ __launch_bounds__(64 * TY) void k_eta_march(Par P, Bufs<TE, TM> B, in
                         const unsigned long long key = (unsigned long long)(key0 + (long long)g * P.ncell + c);
                         bad = bad < key ? bad : key;
                     }
-                    eout[(long long)g * P.ncell + c] = (TE)v;
+                    eout[(long long)g * P.fstride + c] = (TE)v;
                     const double q = v * v;
                     pr += S2 * q;
                     S2 += q;
@@ -899,7 +968,7 @@ __global__ __launch_bounds__(64 * TY) void k_eta_march(Par P, Bufs<TE, TM> B, in
                         const unsigned long long key = (unsigned long long)(key0 + (long long)g * P.ncell + c);
                         bad = bad < key ? bad : key;
                     }
-                    eout[(long long)g * P.ncell + c] = (TE)v;
+                    eout[(long long)g * P.fstride + c] = (TE)v;
                     const float q = v * v;
                     pr += S2 * q;
                     S2 += q;
