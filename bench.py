@@ -58,6 +58,20 @@ def make_model(nicr, shape, n_eta, precision, state):
     return m
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter runs
+    (profiles/traffic_<kernel>_256cube_32eta.json: FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by the same counter's reading for a copy of known size),
+    or None when no such measurement is committed."""
+    path = os.path.join(REPO, "profiles", f"traffic_{kernel}_256cube_32eta.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return {"bytes_per_launch": d["hbm_bytes_per_dispatch"], "read": d["read_bytes_per_dispatch"],
+            "write": d["write_bytes_per_dispatch"], "source": os.path.relpath(path, REPO)}
+
+
 def time_steps(m, steps, warmup, label):
     r = m.try_step(warmup)
     if r.status != 0:
@@ -148,11 +162,18 @@ def main():
                 "kernel_ms": {k: v[0] for k, v in kt64.items()}}
     del state
 
-    # roofline of the dominant kernel (k_eta): algorithmic bytes = each eta read once and written once
-    eta_ms = kt.get("k_eta", (float("nan"), 0))[0]
-    eta_bytes = cells * args.eta * 4 * 2
+    # Roofline of the dominant kernel, k_eta_march (one launch per pass of eta
+    # fields). Algorithmic bytes per launch: 8 B per eta field per cell (FP32
+    # read once + written once) x the fields of one pass x the cells; duration:
+    # mean per launch from HIP events on the context stream.
+    eta_ms, eta_count = kt.get("k_eta_march", (float("nan"), 0))
+    launches_per_step = eta_count / args.steps if eta_count else float("nan")
+    eta_bytes = cells * 8 * args.eta / launches_per_step
     achieved = eta_bytes / (eta_ms * 1e-3) / 1e9
+    pde_ms = kt.get("k_pde_march", (float("nan"), 0))[0]
+    pde_bytes = cells * 56   # o, cr, phi FP64 read + write, gb and S FP32 read
     step_bytes = cells * (48 + 8 * args.eta)   # FP64 o/cr/phi + FP32 eta, read + write
+    traffic = measured_traffic("k_eta_march")
     out = {
         "metric": "Gcell-steps/s @256^3 mixed vs FP64 GPU, % HBM peak",
         "value": round(value, 4),
@@ -171,8 +192,15 @@ def main():
                    "cells": cells, "n_eta": args.eta},
         "step_hbm_fraction": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v[0], 5) for k, v in kt.items()},
-        "roofline": {"bound": "hbm", "kernel": "k_eta", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "kernel_launches_per_step": {k: round(v[1] / args.steps, 3) for k, v in kt.items()},
+        "roofline": {"bound": "hbm", "kernel": "k_eta_march", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes_per_launch": int(eta_bytes), "ms_per_launch": round(eta_ms, 5),
+                     "traffic": traffic},
+        "roofline_pde": {"bound": "hbm", "kernel": "k_pde_march",
+                         "achieved": round(pde_bytes / (pde_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(pde_bytes / (pde_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes_per_launch": int(pde_bytes)},
         "fp64": fp64,
         "copy_probe_GBs": round(copy_gbs, 1),
     }

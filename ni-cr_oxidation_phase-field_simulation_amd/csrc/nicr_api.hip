@@ -39,6 +39,9 @@ struct TimerSlot {
     long long count = 0;
 };
 
+// timer slots: one record per kernel launch
+enum { T_MU, T_TRIAL, T_PDE_MARCH, T_ETA, T_ETA_MARCH, T_COMMIT };
+
 struct Ctx {
     nicr_config cfg{};
     Par P{};
@@ -78,10 +81,11 @@ struct Ctx {
     int eta_ty = 8;                // Y tile height of the marching eta kernel (NICR_ETA_TY: 2, 4, 8)
     int eta_nseg = 0;              // X segments per column (NICR_ETA_NSEG; 0: one wave of resident blocks)
     int eta_ng = 16;               // max eta fields per marching pass (NICR_ETA_NG: 8, 16, 32)
+    bool eta_rows_disabled = false;  // NICR_ETA_ROWS=0 keeps the 64-wide tiles where whole rows would fit
     size_t bytes = 0;
     std::vector<void*> allocs;
     bool timing = false;
-    TimerSlot timers[4] = {{"k_mu"}, {"k_trial"}, {"k_eta"}, {"k_commit"}};
+    TimerSlot timers[6] = {{"k_mu"}, {"k_trial"}, {"k_pde_march"}, {"k_eta"}, {"k_eta_march"}, {"k_commit"}};
     std::vector<hipEvent_t> evpool;
     std::vector<std::pair<int, int>> pending;  // (slot, event index of start)
     size_t evnext = 0;
@@ -174,11 +178,59 @@ static int pick_nseg(int tiles, int resident, int nx, double lead) {
     return best;
 }
 
+// Whole-row variant (k_eta_rows): blocks of 512 / nz rows of nz cells.
+template <int PREC, int NG, int NZ, class TE, class TM>
+void launch_eta_rows_ng(Ctx* c, const Bufs<TE, TM>& B, int g0) {
+    const Par& P = c->P;
+    constexpr int TYR = 512 / NZ;
+    const int tiles = (P.ny + TYR - 1) / TYR;
+    const size_t lds = (size_t)NG * (TYR + 2) * NZ * sizeof(TE);
+    static int resident = 0;
+    if (!resident) {
+        (void)hipFuncSetAttribute((const void*)k_eta_rows<PREC, NG, NZ, TE, TM>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        int per_cu = 0, ncu = 0, dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eta_rows<PREC, NG, NZ, TE, TM>, 512, lds);
+        resident = std::max(1, per_cu) * std::max(1, ncu);
+    }
+    int nseg = pick_nseg(tiles, resident, P.nx, 1.);
+    if (c->eta_nseg > 0) nseg = std::min(c->eta_nseg, P.nx);
+    const int XS = (P.nx + nseg - 1) / nseg;
+    nseg = (P.nx + XS - 1) / XS;
+    tmark_begin(c, T_ETA_MARCH);
+    hipLaunchKernelGGL((k_eta_rows<PREC, NG, NZ, TE, TM>), dim3(tiles * nseg), dim3(512), lds, c->stream, P, B, XS,
+                       g0);
+    tmark_end(c);
+}
+// Whether k_eta_rows applies: nz of 64, 128 or 256, the halo rows x fields split
+// evenly over the 512 / nz thread rows, and the tile fits LDS.
+bool eta_rows_apply(const Ctx* c, int ng, size_t te) {
+    const int nz = c->P.nz;
+    if (c->eta_rows_disabled || (nz != 64 && nz != 128 && nz != 256)) return false;
+    const int ty = 512 / nz;
+    return (2 * ng) % ty == 0 && (size_t)ng * (ty + 2) * nz * te <= 160 * 1024;
+}
+
 // X-marching eta kernel for 3D meshes with up to 32 eta fields: one pass over
 // fields g0 .. g0+NG-1.
 template <int PREC, int NG, int TY, class TE, class TM>
 void launch_eta_march_ng(Ctx* c, const Bufs<TE, TM>& B, int g0) {
     const Par& P = c->P;
+    if (eta_rows_apply(c, NG, sizeof(TE))) {
+        // the thread rows of a block, 512 / nz, must divide 2 * NG
+        if constexpr ((2 * NG) % 2 == 0) {
+            if (P.nz == 256) return launch_eta_rows_ng<PREC, NG, 256>(c, B, g0);
+        }
+        if constexpr ((2 * NG) % 4 == 0) {
+            if (P.nz == 128) return launch_eta_rows_ng<PREC, NG, 128>(c, B, g0);
+        }
+        if constexpr ((2 * NG) % 8 == 0) {
+            if (P.nz == 64) return launch_eta_rows_ng<PREC, NG, 64>(c, B, g0);
+        }
+    }
+
     const int nTz = (P.nz + 63) / 64, nTy = (P.ny + TY - 1) / TY;
     const int tiles = nTz * nTy;
     const size_t lds = (size_t)NG * (TY + 2) * 66 * sizeof(TE);
@@ -198,15 +250,18 @@ void launch_eta_march_ng(Ctx* c, const Bufs<TE, TM>& B, int g0) {
     if (c->eta_nseg > 0) nseg = std::min(c->eta_nseg, P.nx);
     const int XS = (P.nx + nseg - 1) / nseg;
     nseg = (P.nx + XS - 1) / XS;
+    tmark_begin(c, T_ETA_MARCH);
     hipLaunchKernelGGL((k_eta_march<PREC, NG, TY, TE, TM>), dim3(tiles * nseg), dim3(64 * TY), lds, c->stream, P, B,
                        XS, g0);
+    tmark_end(c);
 }
 template <int PREC, int TY, class TE, class TM>
-void launch_eta_march_ty(Ctx* c, const Bufs<TE, TM>& B) {
-    // Passes of exactly NG fields each: the largest NG <= eta_ng (passes keep
-    // the per-XCD plane working set inside L2) that fits the fields left, then
-    // smaller powers of two for the remainder.
-    const int n = c->P.n_eta, cap = c->eta_ng;
+void launch_eta_march_ty(Ctx* c, const Bufs<TE, TM>& B, int cap) {
+    // Passes of exactly NG fields each: the largest NG <= cap (passes keep the
+    // per-XCD plane working set inside L2, and the registers of a binary64 pass
+    // in bounds) that fits the fields left, then smaller powers of two for the
+    // remainder.
+    const int n = c->P.n_eta;
     for (int g0 = 0; g0 < n;) {
         const int left = n - g0;
         if (left >= 32 && cap >= 32) launch_eta_march_ng<PREC, 32, TY>(c, B, g0), g0 += 32;
@@ -220,21 +275,24 @@ void launch_eta_march_ty(Ctx* c, const Bufs<TE, TM>& B) {
 template <int PREC, class TE, class TM>
 bool launch_eta_march(Ctx* c, const Bufs<TE, TM>& B) {
     const int n = c->P.n_eta;
+    // binary64 passes hold at most 8 fields (register planes of more limit
+    // occupancy to one wave per SIMD)
+    const int cap = PREC == PREC_FP64 ? std::min(c->eta_ng, 8) : c->eta_ng;
     // the kernel addresses the fields of one pass with 32-bit byte offsets
     int ng_max = 1;
-    while (ng_max * 2 <= std::min(n, c->eta_ng) && ng_max < 32) ng_max *= 2;
+    while (ng_max * 2 <= std::min(n, cap) && ng_max < 32) ng_max *= 2;
     if (c->march_disabled || c->dim != 3 || n > 32 || c->P.nz < 32 ||
         (unsigned long long)c->P.fstride * sizeof(TE) * (unsigned long long)ng_max >= (1ull << 32))
         return false;
     if constexpr (PREC == PREC_FP64) {
-        // binary64 register planes of more than 8 fields limit occupancy to one
-        // wave per SIMD; the unblocked kernel is faster there
-        if (n > 8) return false;
-        launch_eta_march_ty<PREC, 2>(c, B);
+        // more than 8 fields in binary64: several passes, only with whole-row
+        // tiles (with 64-wide tiles the unblocked kernel is faster there)
+        if (n > 8 && !eta_rows_apply(c, 8, sizeof(TE))) return false;
+        launch_eta_march_ty<PREC, 2>(c, B, cap);
     } else {
-        if (c->eta_ty == 2) launch_eta_march_ty<PREC, 2>(c, B);
-        else if (c->eta_ty == 8) launch_eta_march_ty<PREC, 8>(c, B);
-        else launch_eta_march_ty<PREC, 4>(c, B);
+        if (c->eta_ty == 2) launch_eta_march_ty<PREC, 2>(c, B, cap);
+        else if (c->eta_ty == 8) launch_eta_march_ty<PREC, 8>(c, B, cap);
+        else launch_eta_march_ty<PREC, 4>(c, B, cap);
     }
     return true;
 }
@@ -264,7 +322,9 @@ int launch_pde_march_ty(Ctx* c, const Bufs<TE, TM>& B) {
     const int XS = (P.nx + nseg - 1) / nseg;
     nseg = (P.nx + XS - 1) / XS;
     if ((long long)tiles * nseg > c->partial_slots) return -1;
+    tmark_begin(c, T_PDE_MARCH);
     hipLaunchKernelGGL((k_pde_march<PREC, TY, TE, TM>), dim3(tiles * nseg), dim3(64 * TY), lds, c->stream, P, B, XS);
+    tmark_end(c);
     return tiles * nseg;
 }
 bool pde_march_applies(const Ctx* c) { return !c->march_disabled && c->dim == 3 && c->P.nz >= 32; }
@@ -283,30 +343,27 @@ void launch_step(Ctx* c) {
     auto B = bufs<TE, TM>(c);
     int nparts = -1;
     if constexpr (DIM == 3) {
-        if (pde_march_applies(c)) {
-            tmark_begin(c, 1);
-            nparts = launch_pde_march<PREC, TE, TM>(c, B);
-            tmark_end(c);
-        }
+        if (pde_march_applies(c)) nparts = launch_pde_march<PREC, TE, TM>(c, B);
     }
     if (nparts < 0) {
-        tmark_begin(c, 0);
+        tmark_begin(c, T_MU);
         hipLaunchKernelGGL((k_mu<DIM, PREC, TE, TM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P, B);
         tmark_end(c);
-        tmark_begin(c, 1);
+        tmark_begin(c, T_TRIAL);
         hipLaunchKernelGGL((k_trial<DIM, PREC, TE, TM>), dim3(c->g_trial), dim3(TILE), 0, c->stream, c->P, B);
         tmark_end(c);
         nparts = c->g_trial;
     }
     if (c->eta_evolves()) {
-        tmark_begin(c, 2);
         bool done = false;
         if constexpr (DIM == 3) done = launch_eta_march<PREC, TE, TM>(c, B);
-        if (!done)
+        if (!done) {
+            tmark_begin(c, T_ETA);
             hipLaunchKernelGGL((k_eta<DIM, PREC, TE, TM>), dim3(c->g_eta), dim3(TILE), 0, c->stream, c->P, B);
-        tmark_end(c);
+            tmark_end(c);
+        }
     }
-    tmark_begin(c, 3);
+    tmark_begin(c, T_COMMIT);
     hipLaunchKernelGGL(k_commit, dim3(1), dim3(1024), 0, c->stream, bufs<double, double>(c), nparts);
     tmark_end(c);
 }
@@ -499,6 +556,7 @@ int nicr_create(const nicr_config* cfg, void** out) {
     if (const char* ns = std::getenv("NICR_ETA_NSEG")) c->eta_nseg = std::atoi(ns);
     if (const char* ng = std::getenv("NICR_ETA_NG")) c->eta_ng = std::atoi(ng);
     if (const char* t = std::getenv("NICR_PDE_TY")) c->pde_ty = std::atoi(t);
+    if (const char* t = std::getenv("NICR_ETA_ROWS")) c->eta_rows_disabled = std::atoi(t) == 0;
     if (const char* t = std::getenv("NICR_PDE_NSEG")) c->pde_nseg = std::atoi(t);
     if (c->cfg.dim == 2) c->cfg.nz = 1;
     if (!(c->cfg.grains && c->cfg.grain_evolution)) c->cfg.n_eta = 0;

@@ -183,12 +183,21 @@ template <>
 __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
 }
+// Stores carry the non-temporal hint (cache-policy bit 1): the step writes
+// each output once and reads it only in the next step, so the lines should not
+// displace the inputs that neighbouring blocks are about to read from L2.
+constexpr int BUF_NT = 2;
 __device__ __forceinline__ void buf_store_raw(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, BUF_NT);
 }
 __device__ __forceinline__ void buf_store_raw(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, double v) {
     using U2 = unsigned __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, (int)voff, (int)soff, BUF_NT);
+}
+// the same hint for plain pointer stores
+template <class T>
+__device__ __forceinline__ void st_nt(T* p, T v) {
+    __builtin_nontemporal_store(v, p);
 }
 template <class T>
 __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, T v) {

@@ -365,9 +365,9 @@ __device__ __forceinline__ void finish_cell(const Par& P, long long c, int x, do
     } else if (onv < -1.e-12 || crv < -1.e-12 || onv + crv > 1. + 1.e-12 || pnv < -1.e-12 || pnv > 1. + 1.e-12) {
         a.bad = a.bad < (unsigned long long)c ? a.bad : (unsigned long long)c;
     }
-    on[c] = onv;
-    crn[c] = crv;
-    pn[c] = pnv;
+    st_nt(&on[c], onv);
+    st_nt(&crn[c], crv);
+    st_nt(&pn[c], pnv);
 }
 
 // Block-reduce the accumulators into this block's partial slot.
@@ -788,6 +788,98 @@ __global__ __launch_bounds__(TILE) void k_eta(Par P, Bufs<TE, TM> B) {
     if (bad != NO_KEY) atomicMin(&ctl->eta_key, bad);
 }
 
+// eta trial of all NG fields of one cell (eta_trial, 2D:347-359, SL:514-526)
+// and the next-step mask sums (2D:332), for the X-marching eta kernels: the Y/Z
+// neighbours sit in LDS (one plane tile of PL values per field; o*: offsets
+// of the cell's neighbours in it), the X neighbours and the cell itself in
+// registers. The same operation order as k_eta in binary64.
+template <int PREC, int NG, class TE, class TM>
+__device__ __forceinline__ void eta_cell_update(const Par& P, const Bufs<TE, TM>& B, const TE* lds, int PL, int oym,
+                                                int oyp, int ozm, int ozp, const TE* prev, const TE* cv,
+                                                const TE* nxv, long long c, int cur_i, int nxt_i, bool first_pass,
+                                                bool last_pass, long long key0, __amdgpu_buffer_rsrc_t rout,
+                                                unsigned fs, unsigned long long& bad) {
+    using T = TE;
+    const unsigned cb = (unsigned)c * (unsigned)sizeof(T);
+    const double pp = B.p[cur_i][c];
+    if constexpr (PREC == PREC_FP64) {
+        const double S = (double)B.s2[cur_i][c];
+        double S2 = first_pass ? 0. : (double)B.s2[nxt_i][c];
+        double pr = first_pass ? 0. : (double)B.pairs[nxt_i][c];
+        const double dtl = P.dt * P.etaL;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const T* L = lds + g * PL;
+            const double ec = (double)cv[g];
+            const double eyp = (double)L[oyp];
+            const double eym = (double)L[oym];
+            const double ezp = (double)L[ozp];
+            const double ezm = (double)L[ozm];
+            const double xy = ((double)nxv[g] - 2. * ec + (double)prev[g]) / (P.dx * P.dx)
+                            + (eyp - 2. * ec + eym) / (P.dy * P.dy);
+            const double lp = xy + (ezp - 2. * ec + ezm) / (P.dz * P.dz);
+            const double dF = ec * (P.etaW * (S - (1. - h64(pp)) + 2. * (S - ec * ec)) + P.etaA * h64(pp));
+            const double v = ec - dtl * (dF - P.etaK * lp);
+            if (!isfinite(v) || v < -1.e-12 || v > 1. + 1.e-12) {
+                const unsigned long long key = (unsigned long long)(key0 + (long long)g * P.ncell + c);
+                bad = bad < key ? bad : key;
+            }
+            buf_store<T>(rout, cb, (unsigned)g * fs, (TE)v);
+            const double q = v * v;
+            pr += S2 * q;
+            S2 += q;
+            if ((g & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+        st_nt(&B.s2[nxt_i][c], (TE)S2);
+        st_nt(&B.pairs[nxt_i][c], (TE)pr);
+        if (last_pass) {
+            const double s16 = 16. * pr;
+            st_nt(&B.gb[nxt_i][c], (TE)(P.eta_scale * (s16 < 1. ? s16 : 1.)));
+        }
+    } else {
+#pragma clang fp contract(fast)
+        const float S = (float)B.s2[cur_i][c];
+        const float hphi = h32((float)pp);
+        const float base = (float)P.etaW * (S - (1.f - hphi)) + 2.f * (float)P.etaW * S;
+        const float A = (float)P.etaA * hphi;
+        const float dtl = (float)(P.dt * P.etaL);
+        const float K = (float)P.etaK;
+        const float W2 = 2.f * (float)P.etaW;
+        float S2 = first_pass ? 0.f : (float)B.s2[nxt_i][c];
+        float pr = first_pass ? 0.f : (float)B.pairs[nxt_i][c];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const T* L = lds + g * PL;
+            const float ec = (float)cv[g];
+            const float eyp = (float)L[oyp];
+            const float eym = (float)L[oym];
+            const float ezp = (float)L[ozp];
+            const float ezm = (float)L[ozm];
+            const float lp = ((float)nxv[g] - 2.f * ec + (float)prev[g]) * P.fdx2i
+                           + (eyp - 2.f * ec + eym) * P.fdy2i + (ezp - 2.f * ec + ezm) * P.fdz2i;
+            const float dF = ec * (base - W2 * ec * ec + A);
+            const float v = ec - dtl * (dF - K * lp);
+            if (!isfinite(v) || v < -1.e-12f || v > 1.f + 1.e-12f) {
+                const unsigned long long key = (unsigned long long)(key0 + (long long)g * P.ncell + c);
+                bad = bad < key ? bad : key;
+            }
+            buf_store<T>(rout, cb, (unsigned)g * fs, (TE)v);
+            const float q = v * v;
+            pr += S2 * q;
+            S2 += q;
+            // keep the scheduler from hoisting the LDS reads of all fields at
+            // once (register pressure): groups of four fields
+            if ((g & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+        st_nt(&B.s2[nxt_i][c], (TE)S2);
+        st_nt(&B.pairs[nxt_i][c], (TE)pr);
+        if (last_pass) {
+            const float s16 = 16.f * pr;
+            st_nt(&B.gb[nxt_i][c], (TE)((float)P.eta_scale * (s16 < 1.f ? s16 : 1.f)));
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_eta_march: the same eta trial, restructured so each eta value crosses HBM
 // once. A block owns a 64 (z) x TY (y) column tile of all NG eta fields and
@@ -825,24 +917,28 @@ __global__ __launch_bounds__(64 * TY) void k_eta_march(Par P, Bufs<TE, TM> B, in
     // cell, so the LDS tile row/column after the last active one is exactly
     // the Y/Z periodic neighbour; they compute and store nothing.
     const long long rc = (long long)(y % P.ny) * P.nz + (z % P.nz);
-    const TE* ein = B.eta[cur_i] + (long long)g0 * P.fstride;
-    TE* eout = B.eta[nxt_i] + (long long)g0 * P.fstride;
-    const int ng = min(NG, P.n_eta - g0);
+    // Every pass holds exactly NG fields (the launcher splits n_eta so), which
+    // keeps the field loops free of guards and fully unrolled.
     const bool first_pass = g0 == 0, last_pass = g0 + NG >= P.n_eta;
     const long long key0 = (long long)g0 * P.ncell;
+    // Buffer loads and stores over the NG fields of this pass (the launcher
+    // guarantees NG * fstride * sizeof(TE) < 2^32): the field offset goes in the
+    // scalar offset and the cell offset in one lane register shared by all
+    // fields, so no 64-bit address is kept per field.
+    const unsigned fs = (unsigned)P.fstride * (unsigned)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<TE*>(B.eta[cur_i] + (long long)g0 * P.fstride), 0, (int)(fs * (unsigned)NG), BUF_DWORD3);
+    const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc(
+        B.eta[nxt_i] + (long long)g0 * P.fstride, 0, (int)(fs * (unsigned)NG), BUF_DWORD3);
+    auto cell_bytes = [](long long cell) { return (unsigned)cell * (unsigned)sizeof(T); };
 
     T prev[NG], cv[NG], nx_[NG];
     {
         const int xp = x0 == 0 ? 0 : x0 - 1;
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            if (g < ng) {
-                prev[g] = ein[(long long)g * P.fstride + xp * plane + rc];
-                cv[g] = ein[(long long)g * P.fstride + (long long)x0 * plane + rc];
-            } else {
-                prev[g] = T(0);
-                cv[g] = T(0);
-            }
+            prev[g] = buf_load<T>(rin, cell_bytes(xp * plane + rc), (unsigned)g * fs);
+            cv[g] = buf_load<T>(rin, cell_bytes((long long)x0 * plane + rc), (unsigned)g * fs);
         }
     }
     // halo cells of the tile: row y0-1, row y0+TY (64 each), columns z0-1, z0+64 (TY each)
@@ -853,16 +949,14 @@ __global__ __launch_bounds__(64 * TY) void k_eta_march(Par P, Bufs<TE, TM> B, in
         const long long xn = (long long)(x == P.nx - 1 ? x : x + 1) * plane;
         // issue the next plane's loads early (register prefetch)
 #pragma unroll
-        for (int g = 0; g < NG; ++g)
-            nx_[g] = g < ng ? ein[(long long)g * P.fstride + xn + rc] : T(0);
+        for (int g = 0; g < NG; ++g) nx_[g] = buf_load<T>(rin, cell_bytes(xn + rc), (unsigned)g * fs);
         // interior of plane x into LDS
 #pragma unroll
-        for (int g = 0; g < NG; ++g)
-            if (g < ng) lds[g * PL + (ty + 1) * W + tz + 1] = cv[g];
+        for (int g = 0; g < NG; ++g) lds[g * PL + (ty + 1) * W + tz + 1] = cv[g];
         // halo of plane x: work items (halo cell, chunk of 8 fields); each item
         // issues 8 independent loads so that many are in flight per lane
         constexpr int GC = NG < 8 ? NG : 8;
-        const int nchunk_g = (ng + GC - 1) / GC;
+        constexpr int nchunk_g = (NG + GC - 1) / GC;
         for (int qq = threadIdx.x; qq < nhalo * nchunk_g; qq += 64 * TY) {
             const int gk = qq / nhalo;
             const int hh = qq - gk * nhalo;
@@ -892,95 +986,115 @@ __global__ __launch_bounds__(64 * TY) void k_eta_march(Par P, Bufs<TE, TM> B, in
 #pragma unroll
             for (int u = 0; u < GC; ++u) {
                 const int g = gk * GC + u;
-                hv[u] = g < ng ? ein[(long long)g * P.fstride + hc] : T(0);
+                hv[u] = buf_load<T>(rin, cell_bytes(hc) + (unsigned)g * fs, 0u);
             }
 #pragma unroll
             for (int u = 0; u < GC; ++u) {
                 const int g = gk * GC + u;
-                if (g < ng) lds[g * PL + ly * W + lz] = hv[u];
+                lds[g * PL + ly * W + lz] = hv[u];
             }
         }
         __syncthreads();
-        if (active) {
-            const int lym = ty;
-            const int lzp = tz + 2, lzm = tz;
-            const long long c = xo + rc;
-            const double pp = B.p[cur_i][c];
-            if constexpr (PREC == PREC_FP64) {
-                const double S = (double)B.s2[cur_i][c];
-                double S2 = first_pass ? 0. : (double)B.s2[nxt_i][c];
-                double pr = first_pass ? 0. : (double)B.pairs[nxt_i][c];
-                const double dtl = P.dt * P.etaL;
+        if (active)
+            eta_cell_update<PREC, NG>(P, B, lds, PL, ty * W + tz + 1, (ty + 2) * W + tz + 1, (ty + 1) * W + tz,
+                                      (ty + 1) * W + tz + 2, prev, cv, nx_, xo + rc, cur_i, nxt_i, first_pass,
+                                      last_pass, key0, rout, fs, bad);
+        __syncthreads();
 #pragma unroll
-                for (int g = 0; g < NG; ++g) {
-                    if (g >= ng) break;
-                    const T* L = lds + g * PL;
-                    const double ec = (double)cv[g];
-                    const double eyp = (double)L[(ty + 2) * W + tz + 1];
-                    const double eym = (double)L[lym * W + tz + 1];
-                    const double ezp = (double)L[(ty + 1) * W + lzp];
-                    const double ezm = (double)L[(ty + 1) * W + lzm];
-                    const double xy = ((double)nx_[g] - 2. * ec + (double)prev[g]) / (P.dx * P.dx)
-                                    + (eyp - 2. * ec + eym) / (P.dy * P.dy);
-                    const double lp = xy + (ezp - 2. * ec + ezm) / (P.dz * P.dz);
-                    const double dF = ec * (P.etaW * (S - (1. - h64(pp)) + 2. * (S - ec * ec)) + P.etaA * h64(pp));
-                    const double v = ec - dtl * (dF - P.etaK * lp);
-                    if (!isfinite(v) || v < -1.e-12 || v > 1. + 1.e-12) {
-                        const unsigned long long key = (unsigned long long)(key0 + (long long)g * P.ncell + c);
-                        bad = bad < key ? bad : key;
-                    }
-                    eout[(long long)g * P.fstride + c] = (TE)v;
-                    const double q = v * v;
-                    pr += S2 * q;
-                    S2 += q;
-                }
-                B.s2[nxt_i][c] = (TE)S2;
-                B.pairs[nxt_i][c] = (TE)pr;
-                if (last_pass) {
-                    const double s16 = 16. * pr;
-                    B.gb[nxt_i][c] = (TE)(P.eta_scale * (s16 < 1. ? s16 : 1.));
-                }
-            } else {
-#pragma clang fp contract(fast)
-                const float S = (float)B.s2[cur_i][c];
-                const float hphi = h32((float)pp);
-                const float base = (float)P.etaW * (S - (1.f - hphi)) + 2.f * (float)P.etaW * S;
-                const float A = (float)P.etaA * hphi;
-                const float dtl = (float)(P.dt * P.etaL);
-                const float K = (float)P.etaK;
-                const float W2 = 2.f * (float)P.etaW;
-                float S2 = first_pass ? 0.f : (float)B.s2[nxt_i][c];
-                float pr = first_pass ? 0.f : (float)B.pairs[nxt_i][c];
-#pragma unroll
-                for (int g = 0; g < NG; ++g) {
-                    if (g >= ng) break;
-                    const T* L = lds + g * PL;
-                    const float ec = (float)cv[g];
-                    const float eyp = (float)L[(ty + 2) * W + tz + 1];
-                    const float eym = (float)L[lym * W + tz + 1];
-                    const float ezp = (float)L[(ty + 1) * W + lzp];
-                    const float ezm = (float)L[(ty + 1) * W + lzm];
-                    const float lp = ((float)nx_[g] - 2.f * ec + (float)prev[g]) * P.fdx2i
-                                   + (eyp - 2.f * ec + eym) * P.fdy2i + (ezp - 2.f * ec + ezm) * P.fdz2i;
-                    const float dF = ec * (base - W2 * ec * ec + A);
-                    const float v = ec - dtl * (dF - K * lp);
-                    if (!isfinite(v) || v < -1.e-12f || v > 1.f + 1.e-12f) {
-                        const unsigned long long key = (unsigned long long)(key0 + (long long)g * P.ncell + c);
-                        bad = bad < key ? bad : key;
-                    }
-                    eout[(long long)g * P.fstride + c] = (TE)v;
-                    const float q = v * v;
-                    pr += S2 * q;
-                    S2 += q;
-                }
-                B.s2[nxt_i][c] = (TE)S2;
-                B.pairs[nxt_i][c] = (TE)pr;
-                if (last_pass) {
-                    const float s16 = 16.f * pr;
-                    B.gb[nxt_i][c] = (TE)((float)P.eta_scale * (s16 < 1.f ? s16 : 1.f));
-                }
-            }
+        for (int g = 0; g < NG; ++g) {
+            prev[g] = cv[g];
+            cv[g] = nx_[g];
         }
+    }
+    if (bad != NO_KEY) atomicMin(&ctl->eta_key, bad);
+}
+
+// ---------------------------------------------------------------------------
+// k_eta_rows: the X-marching eta kernel for meshes whose Z rows fit one block
+// (nz = NZ in 64, 128, 256; TY * NZ = 512). A block owns TY whole Z rows, so the
+// Z neighbours wrap inside its LDS tile and, besides the interior, only the two
+// Y halo rows are loaded, as whole contiguous rows. With 64-wide tiles every Z
+// halo cell is a separate cache-line request: as many requests as the interior,
+// most of them missing L2. Same arithmetic as k_eta_march (eta_cell_update).
+// ---------------------------------------------------------------------------
+template <int PREC, int NG, int NZ, class TE, class TM>
+__global__ __launch_bounds__(512) void k_eta_rows(Par P, Bufs<TE, TM> B, int XS, int g0) {
+    using T = TE;
+    constexpr int TY = 512 / NZ;
+    static_assert((2 * NG) % TY == 0, "halo rows x fields must split evenly over the thread rows");
+    constexpr int HC = 2 * NG / TY;          // halo loads per thread: (halo row, field) pairs
+    extern __shared__ unsigned char lds_raw[];
+    T* lds = reinterpret_cast<T*>(lds_raw);
+    DevCtl* ctl = B.ctl;
+    if (ctl->failed) return;
+    const int cur_i = ctl->cur, nxt_i = cur_i ^ 1;
+    constexpr int nz = NZ;
+    constexpr int PL = (TY + 2) * nz;        // per field: halo row y0-1, TY rows, halo row y0+TY
+    const int tid = threadIdx.x;
+    const int ty = tid / nz, z = tid - ty * nz;
+    const int nTy = (P.ny + TY - 1) / TY, nSeg = (P.nx + XS - 1) / XS;
+    const int b = xcd_remap(blockIdx.x, nTy * nSeg);
+    const int seg = b / nTy, tyT = b - seg * nTy;
+    const int y0 = tyT * TY, y = y0 + ty;
+    const bool active = y < P.ny;
+    const int x0 = seg * XS, x1 = min(P.nx, x0 + XS);
+    const long long plane = P.nyz;
+    // rows past the mesh edge of a partial tile hold the periodically wrapped
+    // row, so the tile row after the last active one is its Y neighbour
+    const long long rc = (long long)(y % P.ny) * nz + z;
+    const bool first_pass = g0 == 0, last_pass = g0 + NG >= P.n_eta;
+    const long long key0 = (long long)g0 * P.ncell;
+    // buffer addressing as in k_eta_march
+    const unsigned fs = (unsigned)P.fstride * (unsigned)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<TE*>(B.eta[cur_i] + (long long)g0 * P.fstride), 0, (int)(fs * (unsigned)NG), BUF_DWORD3);
+    const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc(
+        B.eta[nxt_i] + (long long)g0 * P.fstride, 0, (int)(fs * (unsigned)NG), BUF_DWORD3);
+    auto cell_bytes = [](long long cell) { return (unsigned)cell * (unsigned)sizeof(T); };
+    // halo work of this thread: pairs ty + k*TY = (halo row: bit 0, field: the rest)
+    const long long hrow0 = (long long)(((y0 - 1) % P.ny + P.ny) % P.ny) * nz + z;
+    const long long hrow1 = (long long)((y0 + TY) % P.ny) * nz + z;
+    // LDS offsets of the cell's Y/Z neighbours within a field tile
+    const int own = (ty + 1) * nz + z;
+    const int oym = own - nz, oyp = own + nz;
+    const int ozm = (ty + 1) * nz + (z == 0 ? nz - 1 : z - 1);
+    const int ozp = (ty + 1) * nz + (z == nz - 1 ? 0 : z + 1);
+
+    T prev[NG], cv[NG], nx_[NG];
+    {
+        const int xp = x0 == 0 ? 0 : x0 - 1;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            prev[g] = buf_load<T>(rin, cell_bytes(xp * plane + rc), (unsigned)g * fs);
+            cv[g] = buf_load<T>(rin, cell_bytes((long long)x0 * plane + rc), (unsigned)g * fs);
+        }
+    }
+    unsigned long long bad = NO_KEY;
+    for (int x = x0; x < x1; ++x) {
+        const long long xo = (long long)x * plane;
+        const long long xn = (long long)(x == P.nx - 1 ? x : x + 1) * plane;
+        // halo rows of plane x, then the next plane's interior
+        // (a wave lies inside one row, as nz is a multiple of 64: ty, and so the
+        // field of each pair, is uniform over the wave)
+        T hv[HC];
+#pragma unroll
+        for (int k = 0; k < HC; ++k) {
+            const int pr = __builtin_amdgcn_readfirstlane(ty + k * TY), g = pr >> 1;
+            hv[k] = buf_load<T>(rin, cell_bytes(xo + ((pr & 1) ? hrow1 : hrow0)), (unsigned)g * fs);
+        }
+#pragma unroll
+        for (int g = 0; g < NG; ++g) nx_[g] = buf_load<T>(rin, cell_bytes(xn + rc), (unsigned)g * fs);
+#pragma unroll
+        for (int g = 0; g < NG; ++g) lds[g * PL + own] = cv[g];
+#pragma unroll
+        for (int k = 0; k < HC; ++k) {
+            const int pr = ty + k * TY, g = pr >> 1;
+            lds[g * PL + ((pr & 1) ? (TY + 1) * nz : 0) + z] = hv[k];
+        }
+        __syncthreads();
+        if (active)
+            eta_cell_update<PREC, NG>(P, B, lds, PL, oym, oyp, ozm, ozp, prev, cv, nx_, xo + rc, cur_i, nxt_i,
+                                      first_pass, last_pass, key0, rout, fs, bad);
         __syncthreads();
 #pragma unroll
         for (int g = 0; g < NG; ++g) {

@@ -86,6 +86,12 @@ CASES = [
     (3, (8, 7, 64), 0, False, dict(bc="mu", bounds="legacy", ck="off")),
     (3, (5, 3, 33), 0, True, dict(bc="closed", bounds="legacy", ck="legacy", dt=2e-10)),
     (3, (21, 11, 96), 3, True, dict(bc="concentration", bounds="legacy", oxygen_diffusion=False)),
+    # nz of 64, 128 or 256 selects whole-row eta tiles (binary64: passes of 8
+    # fields); partial row tiles wrap periodically, remainder passes of 1 or 2
+    # fields fall back to 64-wide tiles
+    (3, (6, 10, 128), 12, True, dict(bc="concentration", bounds="legacy")),
+    (3, (4, 3, 256), 18, True, dict(bc="mu", bounds="legacy", dt=2e-10)),
+    (3, (5, 2, 64), 32, True, dict(bc="closed", bounds="legacy", ck="off")),
 ]
 
 
@@ -208,7 +214,8 @@ def test_determinism(nicr):
     assert outs[0][-1] == outs[1][-1]
 
 
-@pytest.mark.parametrize("shape,n_eta", [((6, 9, 70), 5), ((7, 3, 40), 9), ((12, 10, 64), 32)])
+@pytest.mark.parametrize("shape,n_eta", [((6, 9, 70), 5), ((7, 3, 40), 9), ((12, 10, 64), 32), ((5, 5, 128), 12),
+                                         ((4, 7, 256), 18)])
 def test_march_equals_basic_kernel(nicr, shape, n_eta, monkeypatch):
     """The X-marching eta kernel and the unblocked kernel give the same fields
     (bitwise in FP64; to binary32 rounding in the mixed path)."""
