@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define NICR_ABI_VERSION 1
+#define NICR_ABI_VERSION 2
 
 /* Return codes. */
 #define NICR_OK 0
@@ -94,6 +94,11 @@ typedef struct nicr_config {
     int32_t reserved0;
     /* Derived eta coefficients (mpi2d:88-103); filled by nicr_config_derive(). */
     double eta_K, eta_A, eta_scale;
+    /* Inlet O concentration of the concentration BC: Physics::layerO = 0.0023
+     * (mpi2d:69, used at mpi2d:216-221,408-411). Not a reference option: the
+     * default is the reference's constant; ensembles vary it per instance as the
+     * O supply. */
+    double layer_o;
 } nicr_config;
 
 /* Budget (mpi2d:193, mpi3d-slab:217). The reference keeps long double sums; the
@@ -175,6 +180,40 @@ int nicr_sync(void* ctx);
  * step kernels' HBM byte counters and roofline fractions are calibrated. No
  * counterpart in the reference. */
 int nicr_copy_probe(void* ctx, size_t bytes, int reps, double* ms);
+
+/* ---- ensembles of 2D instances ----
+ * No counterpart in the reference, whose parameter sweeps are separate runs of
+ * the 2D program (one Model per process, mpi2d:1148-1236). An ensemble holds n
+ * independent 2D instances that share one nicr_config (physics, dt, BCs,
+ * binary64) but differ in mesh size and inlet O supply; every instance steps
+ * exactly as its own nicr context would (bitwise), with its own accepted state,
+ * rejection and budgets. All instances advance together: one persistent kernel
+ * over the tiles of all instances plus a per-instance acceptance kernel per
+ * step, replayed from a captured HIP graph. Evolving grains (eta) and Ksp
+ * insertion are not part of ensembles; the fields are O, CR, PHI and GB. */
+typedef struct nicr_batch_instance {
+    int32_t nx, ny;           /* instance mesh */
+    double layer_o;           /* inlet O of the concentration BC (reference: 0.0023) */
+} nicr_batch_instance;
+
+int nicr_batch_create(const nicr_config* common, const nicr_batch_instance* instances, int n, void** batch);
+int nicr_batch_destroy(void* batch);
+/* Interior cells summed over the instances; device bytes held. */
+int nicr_batch_info(void* batch, long long* cells, size_t* device_bytes);
+/* Fields of instance `inst` (0-based), interior cells in Model order (x outer). */
+int nicr_batch_upload_field(void* batch, int inst, int field, const double* host);
+int nicr_batch_download_field(void* batch, int inst, int field, double* host);
+int nicr_batch_get_budget(void* batch, int inst, nicr_budget* out);
+int nicr_batch_set_budget(void* batch, int inst, const nicr_budget* in);
+/* Up to nsteps steps of every instance; an instance stops at its first rejected
+ * step with the semantics of nicr_step. reports: n entries, or NULL. */
+int nicr_batch_step(void* batch, int nsteps, nicr_step_report* reports);
+/* Tiling: tile height tx (1, 2, 4 or 8 rows of 64 cells), persistent grid size
+ * (0: resident blocks), graph replay on (1) or plain launches (0). */
+int nicr_batch_set_tiling(void* batch, int tx, int grid, int use_graph);
+/* Time every tiling candidate on `steps` dry steps (the instances do not
+ * change) and keep the fastest; returns it and its ms per ensemble step. */
+int nicr_batch_autotune(void* batch, int steps, int* tx, int* grid, double* ms_per_step);
 
 #ifdef __cplusplus
 }

@@ -57,6 +57,7 @@ class NicrConfig(ctypes.Structure):
         ("max_eta_fields", ctypes.c_int32), ("n_eta", ctypes.c_int32), ("precision", ctypes.c_int32),
         ("reserved0", ctypes.c_int32),
         ("eta_K", ctypes.c_double), ("eta_A", ctypes.c_double), ("eta_scale", ctypes.c_double),
+        ("layer_o", ctypes.c_double),
     ]
 
 
@@ -374,3 +375,102 @@ class InitialState:
     center_seed_cells: int
     seeds: np.ndarray
     volumes: np.ndarray
+
+
+class NicrBatchInstance(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("layer_o", ctypes.c_double)]
+
+
+class Ensemble:
+    """Independent 2D instances stepped together on the device (nicr_batch_*).
+
+    `cfg` is the shared 2D configuration (physics, dt, BCs; binary64, no
+    evolving grains); `instances` is a list of (nx, ny, layer_o). Each instance
+    steps bitwise as its own Model would, with its own rejection and budgets.
+    """
+
+    def __init__(self, cfg: NicrConfig, instances):
+        lib = load_library()
+        for name, args in (("create", [ctypes.POINTER(NicrConfig), ctypes.POINTER(NicrBatchInstance), ctypes.c_int,
+                                       ctypes.POINTER(_VOIDP)]),
+                           ("destroy", [_VOIDP]),
+                           ("info", [_VOIDP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_size_t)]),
+                           ("upload_field", [_VOIDP, ctypes.c_int, ctypes.c_int, _DPTR]),
+                           ("download_field", [_VOIDP, ctypes.c_int, ctypes.c_int, _DPTR]),
+                           ("get_budget", [_VOIDP, ctypes.c_int, ctypes.POINTER(NicrBudget)]),
+                           ("set_budget", [_VOIDP, ctypes.c_int, ctypes.POINTER(NicrBudget)]),
+                           ("step", [_VOIDP, ctypes.c_int, ctypes.POINTER(NicrStepReport)]),
+                           ("set_tiling", [_VOIDP, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+                           ("autotune", [_VOIDP, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                         ctypes.POINTER(ctypes.c_int), _DPTR])):
+            fn = getattr(lib, f"nicr_batch_{name}")
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        self.lib = lib
+        self.cfg = cfg
+        self.shapes = [(int(nx), int(ny)) for nx, ny, _ in instances]
+        arr = (NicrBatchInstance * len(instances))(*[NicrBatchInstance(int(nx), int(ny), float(lo))
+                                                      for nx, ny, lo in instances])
+        h = _VOIDP()
+        self._check(lib.nicr_batch_create(ctypes.byref(cfg), arr, len(instances), ctypes.byref(h)))
+        self.h = h
+        self.n = len(instances)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib.nicr_last_error().decode())
+
+    @property
+    def cells(self) -> int:
+        n = ctypes.c_longlong(0)
+        self._check(self.lib.nicr_batch_info(self.h, ctypes.byref(n), None))
+        return n.value
+
+    def get(self, inst: int, name: str) -> np.ndarray:
+        out = np.empty(self.shapes[inst], dtype=np.float64)
+        self._check(self.lib.nicr_batch_download_field(self.h, inst, FIELD[name], out.ctypes.data_as(_DPTR)))
+        return out
+
+    def set(self, inst: int, name: str, arr) -> None:
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(arr, dtype=np.float64), self.shapes[inst]))
+        self._check(self.lib.nicr_batch_upload_field(self.h, inst, FIELD[name], a.ctypes.data_as(_DPTR)))
+
+    def budget(self, inst: int) -> dict:
+        bb = NicrBudget()
+        self._check(self.lib.nicr_batch_get_budget(self.h, inst, ctypes.byref(bb)))
+        return budget_to_dict(bb)
+
+    def budget_raw(self, inst: int) -> NicrBudget:
+        bb = NicrBudget()
+        self._check(self.lib.nicr_batch_get_budget(self.h, inst, ctypes.byref(bb)))
+        return bb
+
+    def set_budget_raw(self, inst: int, bb: NicrBudget) -> None:
+        self._check(self.lib.nicr_batch_set_budget(self.h, inst, ctypes.byref(bb)))
+
+    def step(self, n: int):
+        """n steps of every instance; returns the per-instance NicrStepReport list."""
+        reps = (NicrStepReport * self.n)()
+        self._check(self.lib.nicr_batch_step(self.h, int(n), reps))
+        return list(reps)
+
+    def set_tiling(self, tx: int, grid: int = 0, graph: bool = True) -> None:
+        self._check(self.lib.nicr_batch_set_tiling(self.h, int(tx), int(grid), int(bool(graph))))
+
+    def autotune(self, steps: int = 5):
+        """Pick the fastest tiling on dry steps; returns (tx, grid, ms per step)."""
+        tx, grid, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
+        self._check(self.lib.nicr_batch_autotune(self.h, int(steps), ctypes.byref(tx), ctypes.byref(grid),
+                                                 ctypes.byref(ms)))
+        return tx.value, grid.value, ms.value
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.nicr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

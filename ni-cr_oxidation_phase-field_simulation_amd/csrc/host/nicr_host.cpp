@@ -2,6 +2,7 @@
 // See nicr_host.hpp. Output formats (key order, precision, array order) follow
 // the reference byte for byte; they are interfaces of the programs.
 #include "nicr_host.hpp"
+#include "../../../include/nicr_host.h"
 
 #include <zlib.h>
 

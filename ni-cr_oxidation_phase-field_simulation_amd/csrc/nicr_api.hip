@@ -512,6 +512,7 @@ int nicr_config_default(nicr_config* cfg, int dim) {
     cfg->max_eta_fields = 512;
     cfg->n_eta = 0;
     cfg->precision = NICR_PRECISION_FP64;
+    cfg->layer_o = Phys::layerO;
     return nicr_config_derive(cfg);
 }
 
@@ -591,6 +592,7 @@ int nicr_create(const nicr_config* cfg, void** out) {
     P.gbfc = c->cfg.gb_factor_cr;
     P.mu_res = c->cfg.mu_res;
     P.c_ref = c->cfg.surface_c_ref;
+    P.layer_o = c->cfg.layer_o;
     P.etaW = c->cfg.eta_W;
     P.etaK = c->cfg.eta_K;
     P.etaL = c->cfg.eta_L;
@@ -1080,3 +1082,5 @@ int nicr_sync(void* h) {
 }
 
 }  // extern "C"
+
+#include "nicr_batch.hip"

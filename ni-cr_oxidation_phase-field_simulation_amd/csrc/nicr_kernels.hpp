@@ -74,6 +74,7 @@ struct Par {
     int n_eta, bc, ck_legacy, bounds_legacy;
     int grains, oxy, has_eta, eta_evolves;
     int enforce_layer, pad0;
+    double layer_o;               // inlet O of the concentration BC (Phys::layerO by default)
     double dx, dy, dz, dt;
     double gbfo, gbfc, mu_res, c_ref;
     double etaW, etaK, etaL, etaA, eta_scale;

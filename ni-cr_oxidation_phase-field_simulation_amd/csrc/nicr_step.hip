@@ -129,7 +129,7 @@ __device__ __forceinline__ void gather_oc(const Par& P, const double* o, const d
             c7[s] = c7[SC];
             continue;
         }
-        o7[s] = (enf && qx[s] == 0) ? Phys::layerO : o[idx[s]];
+        o7[s] = (enf && qx[s] == 0) ? P.layer_o : o[idx[s]];
         c7[s] = cr[idx[s]];
     }
 }
@@ -333,11 +333,12 @@ struct StepAcc {
 };
 
 // Legacy Ck (2D:567-571), finite check (2D:572-573), bounds: clamp + phi snap
-// (2D:574-580) or rejection (2D:581-589), inlet budget (2D:408-411,553).
-// binary64 in both precisions. Writes the trial values (raw at a failing cell).
+// (2D:574-580) or rejection (2D:581-589), inlet budget (2D:408-411,553) with
+// inlet O layer_o. binary64 in both precisions. Writes the trial values (raw
+// at a failing cell).
 __device__ __forceinline__ void finish_cell(const Par& P, long long c, int x, double o_stored, double oo, double cc,
                                             double pp, double onv, double crv, double pnv, double left_flux,
-                                            StepAcc& a, double* on, double* crn, double* pn) {
+                                            StepAcc& a, double* on, double* crn, double* pn, double layer_o) {
     if (P.ck_legacy) {
         double dO = -Phys::ckO * ((Phys::oxideO - onv) * h64(pnv) - (Phys::oxideO - oo) * h64(pp)) / P.dt;
         double dCr = -Phys::ckCr * ((Phys::oxideCr - crv) * h64(pnv) - (Phys::oxideCr - cc) * h64(pp)) / P.dt;
@@ -347,7 +348,7 @@ __device__ __forceinline__ void finish_cell(const Par& P, long long c, int x, do
         a.ckCr = dd_add(a.ckCr, dCr);
     }
     if (x == 0) {
-        if (P.enforce_layer) a.bc = dd_add(a.bc, Phys::layerO - o_stored);
+        if (P.enforce_layer) a.bc = dd_add(a.bc, layer_o - o_stored);
         if (P.bc == 1 && P.oxy) a.bc = dd_add(a.bc, P.dt * left_flux / P.dx);
     }
     if (!isfinite(onv) || !isfinite(crv) || !isfinite(pnv)) {
@@ -370,9 +371,10 @@ __device__ __forceinline__ void finish_cell(const Par& P, long long c, int x, do
     st_nt(&pn[c], pnv);
 }
 
-// Block-reduce the accumulators into this block's partial slot.
-__device__ __forceinline__ void write_partials(const StepAcc& a, double* partials, DevCtl* ctl) {
-    if (a.bad != NO_KEY) atomicMin(&ctl->pde_key, a.bad);
+// Block-reduce the accumulators into the partial slot w (NPART doubles) and
+// the failing cell into *key.
+__device__ __forceinline__ void block_partials(const StepAcc& a, double* w, unsigned long long* key) {
+    if (a.bad != NO_KEY) atomicMin(key, a.bad);
     __shared__ DD sh[16];
     const DD r0 = block_reduce_dd(a.bc, sh);
     const DD r1 = block_reduce_dd(a.ckO, sh);
@@ -381,10 +383,13 @@ __device__ __forceinline__ void write_partials(const StepAcc& a, double* partial
     const DD r4 = block_reduce_dd(a.clCr, sh);
     const DD r5 = block_reduce_dd(DD{a.n_clip, 0.}, sh);
     if (threadIdx.x == 0) {
-        double* w = partials + (long long)blockIdx.x * NPART;
         w[0] = r0.hi; w[1] = r0.lo; w[2] = r1.hi; w[3] = r1.lo; w[4] = r2.hi; w[5] = r2.lo;
         w[6] = r3.hi; w[7] = r3.lo; w[8] = r4.hi; w[9] = r4.lo; w[10] = r5.hi; w[11] = 0.;
     }
+}
+// ... into this block's partial slot.
+__device__ __forceinline__ void write_partials(const StepAcc& a, double* partials, DevCtl* ctl) {
+    block_partials(a, partials + (long long)blockIdx.x * NPART, &ctl->pde_key);
 }
 
 template <int DIM, int PREC, class TE, class TM>
@@ -431,7 +436,7 @@ __global__ __launch_bounds__(TILE) void k_trial(Par P, Bufs<TE, TM> B) {
         double onv, crv, pnv, left_flux;
         trial_mm<DIM, PREC, T>(P, m, p7, o7[SC], c7[SC], x, S, (double)gb[c], m64O, onv, crv, pnv, left_flux);
         finish_cell(P, c, x, o[c], o7[SC], c7[SC], p7[SC], onv, crv, pnv, left_flux, acc, B.o[nxt], B.cr[nxt],
-                    B.p[nxt]);
+                    B.p[nxt], P.layer_o);
     }
     write_partials(acc, B.partials, ctl);
 }
@@ -535,7 +540,7 @@ __global__ __launch_bounds__(64 * TY) void k_pde_march(Par P, Bufs<TE, TM> B, in
 #pragma unroll
         for (int r = 0; r < R2; ++r)
             if (in2(r)) {
-                ro[r] = sub ? Phys::layerO : bo[off2[r]];
+                ro[r] = sub ? P.layer_o : bo[off2[r]];
                 rc[r] = bc[off2[r]];
             }
     };
@@ -688,7 +693,7 @@ __global__ __launch_bounds__(64 * TY) void k_pde_march(Par P, Bufs<TE, TM> B, in
             double onv, crv, pnv, left_flux;
             trial_mm<3, PREC, T>(P, m, p7, oo, cc, x, S, g_c, m64O, onv, crv, pnv, left_flux);
             finish_cell(P, c, x, o_stored, oo, cc, p7[SC], onv, crv, pnv, left_flux, acc, B.o[nxt], B.cr[nxt],
-                        B.p[nxt]);
+                        B.p[nxt], P.layer_o);
         }
         m_prev = m[SC];
         p_prev = p7[SC];
@@ -1170,7 +1175,7 @@ __global__ __launch_bounds__(TILE) void k_refresh_out(Par P, Bufs<TE, TM> B, dou
 // enforce_legacy_layer before the trial and nothing restores it).
 __global__ void k_apply_layer(Par P, double* const* obuf, const DevCtl* ctl) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < P.nyz) obuf[ctl->cur][i] = Phys::layerO;
+    if (i < P.nyz) obuf[ctl->cur][i] = P.layer_o;
 }
 
 // ---------------------------------------------------------------------------

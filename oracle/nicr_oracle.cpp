@@ -146,7 +146,7 @@ public:
     void enforce_legacy_layer(){
         if(c.bc!=NICR_BC_CONCENTRATION || !c.oxygen_diffusion)return;
         for(int j=1;j<=ny;++j)for(int k=klo();k<=khi();++k){
-            b.bcO+=Phys::layerO-o[ix(1,j,k)];o[ix(1,j,k)]=Phys::layerO;
+            b.bcO+=c.layer_o-o[ix(1,j,k)];o[ix(1,j,k)]=c.layer_o;   // layer_o: Physics::layerO by default
         }
     }
     // 2D:412-434, SL:597-620 (the SL frozen-eta cache is bitwise equal to a rebuild,
