@@ -115,6 +115,77 @@ def cpu_baseline(nicr, n_eta, seed):
                       f"{done} FP64 steps, {t1 - t0:.1f} s on one host thread"}
 
 
+def ensemble_state(shape, cr0, layer_o, rng):
+    """C1-like 2D state: O = 0 with the inlet row at layer_o (Cr 0.182), Cr =
+    cr0 + U(-0.005, 0.005), centre oxide disk of radius 5 cells."""
+    nx, ny = shape
+    o = np.zeros(shape)
+    cr = cr0 + rng.uniform(-0.005, 0.005, size=shape)
+    p = np.zeros(shape)
+    o[0, :] = layer_o
+    cr[0, :] = 0.182
+    i, j = np.meshgrid(np.arange(1, nx + 1), np.arange(1, ny + 1), indexing="ij")
+    disk = (i - (nx + 1) / 2.0) ** 2 + (j - (ny + 1) / 2.0) ** 2 <= 25.0
+    o[disk], cr[disk], p[disk] = 0.56300203, 0.35677915, 1.0
+    return o, cr, p
+
+
+def ensemble_bench(nicr, n_inst, steps, seed):
+    """Ensemble of n_inst 2D instances, sides from {64, 128, 256, 512}, Cr in
+    U(0.15, 0.30), inlet O supply U(0.5, 1.5) x 0.0023: autotuned graph replay vs
+    the same instances stepped one after another as separate contexts."""
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([64, 128, 256, 512], size=n_inst)
+    crs = rng.uniform(0.15, 0.30, size=n_inst)
+    supply = rng.uniform(0.5, 1.5, size=n_inst) * 0.0023
+    insts = [(int(s), int(s), float(lo)) for s, lo in zip(sides, supply)]
+    states = [ensemble_state((s, s), c, lo, rng) for (s, _, lo), c in zip(insts, crs)]
+    cfg = nicr.make_config(2, nx=8, ny=8, grains=False, bc="concentration", bounds="legacy", ck="legacy",
+                           nucleation="off")
+    ens = nicr.Ensemble(cfg, insts)
+    for i, st in enumerate(states):
+        for name, a in zip(("o", "cr", "p"), st):
+            ens.set(i, name, a)
+    cells = ens.cells
+    tx, grid, tune_ms = ens.autotune(5)
+    ens.step(10)
+    t0 = time.perf_counter()
+    reps = ens.step(steps)
+    t1 = time.perf_counter()
+    accepted = sum(r.steps_done for r in reps)
+    ens.set_tiling(tx, grid, graph=False)
+    t2 = time.perf_counter()
+    ens.step(steps)
+    t3 = time.perf_counter()
+    ens.close()
+    # the same instances as separate single-instance contexts, one after another
+    models = []
+    for (nx, ny, lo), st in zip(insts, states):
+        c1 = nicr.make_config(2, nx=nx, ny=ny, grains=False, bc="concentration", bounds="legacy", ck="legacy",
+                              nucleation="off", layer_o=lo)
+        m = nicr.Model(c1)
+        for name, a in zip(("o", "cr", "p"), st):
+            m.set(name, a)
+        m.step(2)
+        models.append(m)
+    t4 = time.perf_counter()
+    for m in models:
+        m.step(steps)
+    t5 = time.perf_counter()
+    for m in models:
+        m.close()
+    return {"workload": f"{n_inst} 2D instances, sides {sorted(set(int(s) for s in sides))}, Cr U(0.15,0.30), "
+                        "inlet O U(0.5,1.5)x0.0023, centre seed r=5, bounds legacy, binary64",
+            "cells": cells, "steps": steps, "all_accepted": accepted == steps * n_inst,
+            "autotuned": {"tx": tx, "grid": grid, "ms_per_step_dry": round(tune_ms, 4)},
+            "graph": {"value": round(cells * steps / (t1 - t0) / 1e9, 4), "ms_per_step": round((t1 - t0) / steps * 1e3, 4)},
+            "eager": {"value": round(cells * steps / (t3 - t2) / 1e9, 4), "ms_per_step": round((t3 - t2) / steps * 1e3, 4)},
+            "sequential_contexts": {"value": round(cells * steps / (t5 - t4) / 1e9, 4),
+                                    "ms_per_step": round((t5 - t4) / steps * 1e3, 4)},
+            "unit": "Gcell-steps/s",
+            "speedup_vs_sequential": round((t5 - t4) / (t1 - t0), 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +196,8 @@ def main():
     ap.add_argument("--seeds", type=int, default=64)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 2D ensemble measurement")
+    ap.add_argument("--ensemble-only", action="store_true", help="only the 2D ensemble measurement")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the 1 GiB device copy reference (rocprof counter calibration)")
     args = ap.parse_args()
@@ -138,6 +211,9 @@ def main():
         m = nicr.Model(cfg)
         ms = m.copy_probe(1 << 30, 20)
         print(json.dumps({"copy_probe_bytes": 1 << 30, "ms": ms, "GBs_read_plus_write": 2 * (1 << 30) / ms / 1e6}))
+        return
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_bench(nicr, 64, 200, 20260909)}), flush=True)
         return
     shape = (args.n, args.n, args.n)
     cells = shape[0] * shape[1] * shape[2]
@@ -204,6 +280,8 @@ def main():
         "fp64": fp64,
         "copy_probe_GBs": round(copy_gbs, 1),
     }
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(nicr, 64, 200, 20260909)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nicr, args.eta, 20260909)
     print(json.dumps(out), flush=True)
