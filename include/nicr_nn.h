@@ -1,0 +1,117 @@
+/* nicr_nn.h -- learned surrogate of the step (part of libnicr.so).
+ *
+ * No counterpart in the reference, which has only the explicit step
+ * (mpi2d/NiCr2D_grains.cpp:551-597, mpi3d-slab/NiCr3D_grains.cpp:761-826). A 3D
+ * U-Net (4 levels, 32/64/128/256 channels, 3x3x3 convolutions with ReLU, 2x2x2
+ * average pooling, nearest upsampling with skip concatenation, FP32 1x1 head;
+ * X zero padded, Y/Z periodic like the mesh) maps the state (O, Cr, phi, gb
+ * mask) to its change over `jump` steps. Weights and activations are BF16 with
+ * FP32 accumulation on MFMA; master weights, gradients and Adam moments are FP32.
+ *
+ * Conventions as nicr.h: plain C, host arrays borrowed for one call, every
+ * function returns NICR_OK or a negative NICR_E_* code (nicr_last_error()).
+ * Host volumes are float32 in the model's cell order ((x*ny + y)*nz + z), one
+ * channel after another: inputs [nb][4][V] (O, Cr, phi, gb), outputs and
+ * targets [nb][3][V] (normalised changes of O, Cr, phi).
+ */
+#ifndef NICR_NN_H
+#define NICR_NN_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "nicr.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A network for volumes of s0 x s1 x s2 cells (multiples of 8), batches of up
+ * to max_batch volumes. train != 0 allocates gradients and Adam moments;
+ * max_samples > 0 reserves a training set of that many samples on the device.
+ * Weights: He-normal from `seed`, zero biases, zero head. */
+int nicr_unet_create(int s0, int s1, int s2, int max_batch, int train, int max_samples, uint64_t seed, void** net);
+int nicr_unet_destroy(void* net);
+int nicr_unet_param_count(void* net, long long* count);
+int nicr_unet_device_bytes(void* net, size_t* bytes);
+/* FP32 parameters: per convolution W[27][cout][cin] then b[cout] (tap t = 9a +
+ * 3b + c has offsets a-1, b-1, c-1 along X, Y, Z) in the order e1a e1b e2a e2b
+ * e3a e3b b4a b4b d3a d3b d2a d2b d1a d1b, then the head W[3][32], b[3].
+ * set_params also resets the Adam state. */
+int nicr_unet_get_params(void* net, float* host);
+int nicr_unet_set_params(void* net, const float* host);
+/* Gradients of the last nicr_unet_loss_backward (same layout as the parameters). */
+int nicr_unet_get_grads(void* net, float* host);
+/* Input normalisation x_c = (f_c - in_mean_c) / in_scale_c and output scale:
+ * the change of field c is out_c * out_scale_c (out_scale_c = 0 freezes c). */
+int nicr_unet_set_norm(void* net, const float in_mean[4], const float in_scale[4], const float out_scale[3]);
+int nicr_unet_get_norm(void* net, float in_mean[4], float in_scale[4], float out_scale[3]);
+/* Forward pass of nb volumes: in [nb][4][V] -> out [nb][3][V] (normalised). */
+int nicr_unet_forward(void* net, int nb, const float* in, float* out);
+/* Forward pass, then the gradients of L = (1/3) sum_p sum_c (out - target)^2
+ * with respect to every parameter (overwritten, not accumulated); *loss = L
+ * divided by the number of positions. */
+int nicr_unet_loss_backward(void* net, int nb, const float* in, const float* target, double* loss);
+/* Device time of the network's forward passes inside nicr_sur_advance. */
+int nicr_unet_set_timing(void* net, int enabled);
+int nicr_unet_forward_ms(void* net, double* ms);
+/* One 3x3x3 convolution of the network's kernels on host arrays, for kernel
+ * checks: op 0 forward y = act(conv(x, w) + b) -> out [P][cout]; op 1 data
+ * gradient conv^T(dy, w) -> out [P][cin]; op 2 weight gradient -> out
+ * [27][cout][cin]. x [P][cin], dy [P][cout] channels-last over P = nb*s0*s1*s2
+ * positions (rounded to BF16 on upload), w [27][cout][cin], b [cout];
+ * cin, cout multiples of 32. */
+int nicr_nn_conv_check(int op, int nb, int s0, int s1, int s2, int cin, int cout, int relu, const float* x,
+                       const float* w, const float* b, const float* dy, float* out);
+
+/* ---- training set and training ---- */
+/* Record one training sample from a binary64 3D model of the network's volume:
+ * its current (O, Cr, phi, mask) and the change after `jump` steps, which this
+ * call takes on the model (nicr_step). A rejected step records nothing and is
+ * reported in *report. */
+int nicr_sur_record(void* net, void* model, int jump, nicr_step_report* report);
+int nicr_sur_samples(void* net, int* count);
+/* Normalisation from the training set (input mean and standard deviation,
+ * output RMS). */
+int nicr_sur_fit_norm(void* net);
+/* `iters` Adam iterations (learning rate lr) on random batches of `batch`
+ * samples drawn with `seed`; losses[iters] (mean squared error of the
+ * normalised changes per iteration) when not NULL. */
+int nicr_sur_train(void* net, int iters, int batch, float lr, uint64_t seed, float* losses);
+/* Mean squared error of the normalised changes over samples [first, first+count). */
+int nicr_sur_eval(void* net, int first, int count, double* mse);
+
+/* ---- surrogate-accelerated stepping ---- */
+typedef struct nicr_sur_options {
+    int32_t jump;             /* steps replaced by one network proposal (50) */
+    int32_t conserve_mass;    /* rescale O and Cr to their pre-jump totals (closed boundaries) */
+    double residual_tol;      /* accept when ||R|| <= residual_tol * ||proposal - x0|| for O and Cr */
+    double min_value, max_value;   /* accept only O, Cr, phi inside [min, max] */
+} nicr_sur_options;
+
+typedef struct nicr_sur_report {
+    int32_t steps_done;       /* model steps advanced (plain + jumps x jump) */
+    int32_t plain_steps;      /* steps taken by the binary64 step itself */
+    int32_t jumps_accepted, jumps_rejected;
+    int32_t status;           /* NICR_STEP_* of a rejected plain step, else OK */
+    int32_t reserved;
+    double max_residual_o, max_residual_cr;   /* largest accepted ||R|| / ||proposal - x0|| */
+    double min_rejected_residual;             /* smallest max(O, Cr) residual among rejections */
+    nicr_step_report step;    /* the plain step's report when status != OK */
+} nicr_sur_report;
+
+/* Advance a binary64 3D model by nsteps with network jumps of `jump` steps.
+ * Each cycle: one plain step x_-1 -> x0 (update u0 = x0 - x_-1); the proposal
+ * x0 + network(x0) for x0 after `jump` steps (eta fields are held); optionally
+ * O and Cr rescaled to their totals at x0; one plain check step from the
+ * proposal xh to xh1. The jump is accepted when O, Cr, phi are inside
+ * [min, max], the model accepts the check step, and for O and Cr the trapezoid
+ * residual R = (xh - x0) - (jump / 2) (u0 + (xh1 - xh)) satisfies
+ * ||R|| <= residual_tol ||xh - x0|| (a proposal that does not move fails).
+ * Otherwise the model is put back exactly to x0 (fields, state index, budgets)
+ * and takes the `jump` plain steps instead. */
+int nicr_sur_advance(void* net, void* model, int nsteps, const nicr_sur_options* opt, nicr_sur_report* report);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NICR_NN_H */

@@ -1084,3 +1084,4 @@ int nicr_sync(void* h) {
 }  // extern "C"
 
 #include "nicr_batch.hip"
+#include "nicr_nn.hip"

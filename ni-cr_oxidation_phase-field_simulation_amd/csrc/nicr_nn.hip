@@ -1,0 +1,1242 @@
+// nicr_nn.hip -- learned surrogate: a 4-level 3D U-Net (BF16 weights and
+// activations, FP32 accumulation) trained on the GPU by its own forward and
+// backward kernels on trajectories of the plain step, and surrogate jumps that
+// the binary64 step accepts or rejects. Included by nicr_api.hip (one library).
+// No counterpart in the reference.
+//
+// Network (channels per level 32, 64, 128, 256; 3x3x3 convolutions, ReLU; X
+// zero padded, Y/Z periodic):
+//   x0 (O, Cr, phi, gb normalised; 4 of 32 channels)
+//   e1 = conv(conv(x0))          p1 = pool(e1)       level 0: 32
+//   e2 = conv(conv(p1))          p2 = pool(e2)       level 1: 64
+//   e3 = conv(conv(p2))          p3 = pool(e3)       level 2: 128
+//   b4 = conv(conv(p3))                              level 3: 256
+//   d3 = conv(conv([up(b4), e3]))                    level 2: 384 -> 128
+//   d2 = conv(conv([up(d3), e2]))                    level 1: 192 -> 64
+//   d1 = conv(conv([up(d2), e1]))                    level 0: 96 -> 32
+//   out = head(d1)   FP32 1x1 projection onto the 3 normalised 50-step deltas
+// Parameters are one FP32 array: per convolution W[27][cout][cin] then b[cout],
+// in the order above (e1a e1b e2a e2b e3a e3b b4a b4b d3a d3b d2a d2b d1a d1b),
+// then the head W[3][32] and b[3].
+#include "../../include/nicr_nn.h"
+#include "nicr_nn_kernels.hpp"
+
+#include <algorithm>
+#include <random>
+
+namespace nicr {
+namespace nn {
+
+constexpr int N_CONV = 14;
+constexpr int HEAD_IN = 32;
+constexpr int N_IN = 4, N_OUT = 3;
+
+// ---- head, loss, inputs, optimiser ----
+
+// out[p][c] = bh[c] + sum_ci D[p][ci] Wh[c][ci] (c < 3; out row pitch 4).
+__global__ void k_head(int P, const bf16* __restrict__ D, const float* __restrict__ Wh, const float* __restrict__ bh,
+                       float* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float s0 = bh[0], s1 = bh[1], s2 = bh[2];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        F8 f;
+        f.from(bf8_load(D + (long long)p * HEAD_IN + 8 * q));
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int ci = 8 * q + i;
+            s0 += f.v[i] * Wh[ci];
+            s1 += f.v[i] * Wh[HEAD_IN + ci];
+            s2 += f.v[i] * Wh[2 * HEAD_IN + ci];
+        }
+    }
+    *reinterpret_cast<float4*>(out + 4LL * p) = make_float4(s0, s1, s2, 0.f);
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Loss L = (1/3) sum_p sum_c (out - t)^2 and its gradient through the head:
+// gD[p][ci] = (sum_c g_c Wh[c][ci]) * (D[p][ci] > 0) with g_c = 2 (out - t) / 3;
+// dWh[c][ci] += sum_p g_c D[p][ci]; dbh[c] += sum_p g_c; loss[0] += sum (out - t)^2.
+constexpr int HB_THREADS = 256;
+__global__ __launch_bounds__(HB_THREADS) void k_head_bwd(int P, const bf16* __restrict__ D,
+                                                         const float* __restrict__ out, const float* __restrict__ tgt,
+                                                         const float* __restrict__ Wh, bf16* __restrict__ gD,
+                                                         float* __restrict__ dWh, float* __restrict__ dbh,
+                                                         float* __restrict__ loss) {
+    constexpr int NACC = N_OUT * HEAD_IN + N_OUT + 1;
+    __shared__ float red[HB_THREADS / 64][NACC];
+    float acc[NACC];
+#pragma unroll
+    for (int i = 0; i < NACC; ++i) acc[i] = 0.f;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+        const float4 o = *reinterpret_cast<const float4*>(out + 4LL * p);
+        const float4 t = *reinterpret_cast<const float4*>(tgt + 4LL * p);
+        const float e0 = o.x - t.x, e1 = o.y - t.y, e2 = o.z - t.z;
+        const float g0 = (2.f / 3.f) * e0, g1 = (2.f / 3.f) * e1, g2 = (2.f / 3.f) * e2;
+        acc[NACC - 1] += e0 * e0 + e1 * e1 + e2 * e2;
+        acc[N_OUT * HEAD_IN + 0] += g0;
+        acc[N_OUT * HEAD_IN + 1] += g1;
+        acc[N_OUT * HEAD_IN + 2] += g2;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            F8 f, gg;
+            f.from(bf8_load(D + (long long)p * HEAD_IN + 8 * q));
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int ci = 8 * q + i;
+                acc[ci] += g0 * f.v[i];
+                acc[HEAD_IN + ci] += g1 * f.v[i];
+                acc[2 * HEAD_IN + ci] += g2 * f.v[i];
+                const float gv = g0 * Wh[ci] + g1 * Wh[HEAD_IN + ci] + g2 * Wh[2 * HEAD_IN + ci];
+                gg.v[i] = f.v[i] > 0.f ? gv : 0.f;
+            }
+            bf8_store(gD + (long long)p * HEAD_IN + 8 * q, gg.to());
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NACC; ++i) {
+        const float v = wave_sum(acc[i]);
+        if (lane == 0) red[wave][i] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < NACC; i += blockDim.x) {
+        float v = 0.f;
+#pragma unroll
+        for (int w = 0; w < HB_THREADS / 64; ++w) v += red[w][i];
+        if (i < N_OUT * HEAD_IN) atomicAdd(&dWh[i], v);
+        else if (i < NACC - 1) atomicAdd(&dbh[i - N_OUT * HEAD_IN], v);
+        else atomicAdd(loss, v);
+    }
+}
+
+struct Norm {
+    float mean[N_IN], inv[N_IN];   // x0 channel c = (field_c - mean_c) * inv_c
+    float out_scale[N_OUT];        // delta_c = out_c * out_scale_c
+    float out_inv[N_OUT];          // target_c = delta_c * out_inv_c (0 where out_scale is 0)
+};
+
+// Network input of one volume: x0[p][c] = (f_c[p] - mean_c) * inv_c for c < 4, 0 above.
+template <class T0, class T3>
+__global__ void k_input(int V, const T0* __restrict__ f0, const T0* __restrict__ f1, const T0* __restrict__ f2,
+                        const T3* __restrict__ f3, Norm nm, bf16* __restrict__ x0) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= V) return;
+    bf16x8 v;
+    v[0] = (bf16)(((float)f0[p] - nm.mean[0]) * nm.inv[0]);
+    v[1] = (bf16)(((float)f1[p] - nm.mean[1]) * nm.inv[1]);
+    v[2] = (bf16)(((float)f2[p] - nm.mean[2]) * nm.inv[2]);
+    v[3] = (bf16)(((float)f3[p] - nm.mean[3]) * nm.inv[3]);
+#pragma unroll
+    for (int i = 4; i < 8; ++i) v[i] = (bf16)0.f;
+    uint4* dst = reinterpret_cast<uint4*>(x0 + 32LL * p);
+    dst[0] = __builtin_bit_cast(uint4, v);
+    dst[1] = dst[2] = dst[3] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Training target of one volume: tgt[p] = (d0, d1, d2) * out_inv, row pitch 4.
+__global__ void k_target(int V, const float* __restrict__ d, Norm nm, float* __restrict__ tgt) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= V) return;
+    *reinterpret_cast<float4*>(tgt + 4LL * p) =
+        make_float4(d[p] * nm.out_inv[0], d[V + p] * nm.out_inv[1], d[2LL * V + p] * nm.out_inv[2], 0.f);
+}
+
+// Surrogate jump: field_c += out[p][c] * out_scale_c (binary64 state).
+__global__ void k_apply_jump(int V, const float* __restrict__ out, Norm nm, double* __restrict__ o,
+                             double* __restrict__ cr, double* __restrict__ ph) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= V) return;
+    const float4 v = *reinterpret_cast<const float4*>(out + 4LL * p);
+    o[p] += (double)v.x * (double)nm.out_scale[0];
+    cr[p] += (double)v.y * (double)nm.out_scale[1];
+    ph[p] += (double)v.z * (double)nm.out_scale[2];
+}
+
+__global__ void k_scale(long long n, double* __restrict__ a, double f) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] *= f;
+}
+
+// acc[0] += sum (a - b)^2, acc[1] += sum a^2 (binary64).
+__global__ __launch_bounds__(256) void k_diff_sq(long long n, const double* __restrict__ a,
+                                                 const double* __restrict__ b, double* __restrict__ acc) {
+    __shared__ double red[2][4];
+    double s = 0., t = 0.;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double d = a[i] - b[i];
+        s += d * d;
+        t += a[i] * a[i];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        t += __shfl_xor(t, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = s;
+        red[1][threadIdx.x >> 6] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&acc[0], red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+        atomicAdd(&acc[1], red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+    }
+}
+
+// Trapezoid residual of a jump of J steps from x0 to xh: R = (xh - x0) -
+// (J/2)(u0 + (xh1 - xh)) with u0 the plain update at x0 and xh1 one plain step
+// from xh. acc[0] += sum R^2, acc[1] += sum (xh - x0)^2.
+__global__ __launch_bounds__(256) void k_trap(long long n, const double* __restrict__ xh,
+                                              const double* __restrict__ xh1, const double* __restrict__ x0,
+                                              const double* __restrict__ u0, double halfJ, double* __restrict__ acc) {
+    __shared__ double red[2][4];
+    double s = 0., t = 0.;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double d = xh[i] - x0[i];
+        const double r = d - halfJ * (u0[i] + (xh1[i] - xh[i]));
+        s += r * r;
+        t += d * d;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        t += __shfl_xor(t, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = s;
+        red[1][threadIdx.x >> 6] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&acc[0], red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+        atomicAdd(&acc[1], red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+    }
+}
+
+__global__ void k_sub(long long n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] - b[i];
+}
+
+// Moments of a float array: acc[0] += sum x, acc[1] += sum x^2 (binary64).
+__global__ __launch_bounds__(256) void k_moments(long long n, const float* __restrict__ x, double* __restrict__ acc) {
+    __shared__ double red[2][4];
+    double s = 0., t = 0.;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double v = x[i];
+        s += v;
+        t += v * v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        t += __shfl_xor(t, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = s;
+        red[1][threadIdx.x >> 6] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&acc[0], red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+        atomicAdd(&acc[1], red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+    }
+}
+
+template <class T>
+__global__ void k_to_float(long long n, const T* __restrict__ a, float* __restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)a[i];
+}
+// out = float(a - b)
+__global__ void k_delta_float(long long n, const double* __restrict__ a, const double* __restrict__ b,
+                              float* __restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)(a[i] - b[i]);
+}
+
+// Adam (bias-corrected) on the FP32 master parameters.
+__global__ void k_adam(long long n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float c1, float c2, float eps) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// BF16 copies of a convolution's weights: Wb[tap][co][ci] and the data-gradient
+// form Wt[taps - 1 - tap][ci][co].
+__global__ void k_cast_w(int taps, int cout, int cin, const float* __restrict__ W, bf16* __restrict__ Wb,
+                         bf16* __restrict__ Wt) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n = (long long)taps * cout * cin;
+    if (i >= n) return;
+    const int ci = (int)(i % cin), co = (int)((i / cin) % cout), tap = (int)(i / ((long long)cin * cout));
+    const bf16 b = (bf16)W[i];
+    Wb[i] = b;
+    Wt[((long long)(taps - 1 - tap) * cin + ci) * cout + co] = b;
+}
+
+// ---- network ----
+
+struct ConvL {
+    int cin, cout, level;
+    bool relu;
+    long long w_off, b_off;   // into the parameter array
+    bf16* Wb = nullptr;
+    bf16* Wt = nullptr;
+    bf16 *x = nullptr, *y = nullptr;      // input and output activations
+    bf16 *gx = nullptr, *gy = nullptr;    // their gradients (gy: pre-activation)
+};
+
+inline unsigned nblk(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+struct Net {
+    int nbmax = 1, S[3] = {0, 0, 0};
+    bool train = false;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    size_t bytes = 0;
+    long long nparam = 0, head_w = 0, head_b = 0;
+    float *param = nullptr, *grad = nullptr, *am = nullptr, *av = nullptr;
+    int adam_t = 0;
+    ConvL conv[N_CONV];
+    // activations
+    bf16 *x0, *e1a, *e1, *p1, *e2a, *e2, *p2, *e3a, *e3, *p3, *b4a, *b4, *u3, *d3a, *d3, *u2, *d2a, *d2, *u1, *d1a,
+        *d1;
+    float *out = nullptr, *tgt = nullptr;
+    // gradients
+    bf16 *g_d1, *g_d1a, *g_u1, *g_d2, *g_d2a, *g_u2, *g_d3, *g_d3a, *g_u3, *g_b4, *g_b4a, *g_p3, *g_e3, *g_e3a, *g_p2,
+        *g_e2, *g_e2a, *g_p1, *g_e1, *g_e1a;
+    float* loss_acc = nullptr;   // one slot per training iteration
+    int loss_cap = 0;
+    double* dacc = nullptr;      // reduction scratch
+    Norm nm{};
+    // training set: inputs [S][4][V] and 50-step deltas [S][3][V] (FP32)
+    int ds_cap = 0, ds_n = 0;
+    float *ds_in = nullptr, *ds_d = nullptr;
+    // binary64 state backup of a surrogate jump (O, Cr, phi)
+    double* bk[3] = {nullptr, nullptr, nullptr};
+    double* du[2] = {nullptr, nullptr};   // plain update of O and Cr before a jump
+    long long bk_cells = 0;
+    bool timing = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double fwd_ms_sum = 0.;
+    int fwd_count = 0;
+
+    long long V(int l) const { return (long long)(S[0] >> l) * (S[1] >> l) * (S[2] >> l); }
+};
+
+int nalloc(Net* n, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(NICR_E_DEVICE, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    n->allocs.push_back(*p);
+    n->bytes += bytes;
+    return NICR_OK;
+}
+template <class T>
+int nalloc_t(Net* n, T** p, long long count) {
+    void* q = nullptr;
+    int rc = nalloc(n, &q, (size_t)count * sizeof(T));
+    *p = static_cast<T*>(q);
+    return rc;
+}
+
+void net_free(Net* n) {
+    for (void* p : n->allocs) hipFree(p);
+    n->allocs.clear();
+    if (n->ev[0]) hipEventDestroy(n->ev[0]);
+    if (n->ev[1]) hipEventDestroy(n->ev[1]);
+    if (n->stream) hipStreamDestroy(n->stream);
+    delete n;
+}
+
+Geo geo(const Net* n, int nb, int level, int cin, int cout, int taps) {
+    Geo g;
+    g.nb = nb;
+    g.s0 = n->S[0] >> level;
+    g.s1 = n->S[1] >> level;
+    g.s2 = n->S[2] >> level;
+    g.cin = cin;
+    g.cout = cout;
+    g.taps = taps;
+    return g;
+}
+
+void launch_conv(Net* n, const Geo& g, const bf16* X, const bf16* W, const float* bias, bf16* Y, bool relu) {
+    const int P = g.positions();
+    const unsigned mb = (unsigned)((P + CBM - 1) / CBM);
+    if (g.cout % 64 == 0) {
+        const dim3 grid(mb * (g.cout / 64));
+        if (relu) hipLaunchKernelGGL((k_conv<64, true>), grid, dim3(256), 0, n->stream, g, X, W, bias, Y);
+        else hipLaunchKernelGGL((k_conv<64, false>), grid, dim3(256), 0, n->stream, g, X, W, bias, Y);
+    } else {
+        const dim3 grid(mb * (g.cout / 32));
+        if (relu) hipLaunchKernelGGL((k_conv<32, true>), grid, dim3(256), 0, n->stream, g, X, W, bias, Y);
+        else hipLaunchKernelGGL((k_conv<32, false>), grid, dim3(256), 0, n->stream, g, X, W, bias, Y);
+    }
+}
+
+void launch_wgrad(Net* n, const Geo& g, const bf16* X, const bf16* dY, float* dW) {
+    const int P = g.positions();
+    const int tiles = (g.cout / 32) * (g.cin / 32);
+    int nchunk = std::max(1, 4096 / (tiles * g.taps));
+    int chunk = (P + nchunk - 1) / nchunk;
+    chunk = std::max(128, (chunk + 127) / 128 * 128);
+    nchunk = (P + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_conv_wgrad, dim3((unsigned)(tiles * g.taps * nchunk)), dim3(256), 0, n->stream, g, X, dY, dW,
+                       chunk);
+}
+
+void launch_bias_grad(Net* n, int P, int C, const bf16* G, float* dB) {
+    const int cg = C / 32;
+    int nchunk = std::max(1, std::min(2048 / cg, (P + 1023) / 1024));
+    const int chunk = (P + nchunk - 1) / nchunk;
+    nchunk = (P + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)(cg * nchunk)), dim3(256), 0, n->stream, P, C, chunk, G, dB);
+}
+
+int net_build(Net* n) {
+    // convolution table: (cin, cout, level)
+    const int tab[N_CONV][3] = {{32, 32, 0},   {32, 32, 0},   {32, 64, 1},   {64, 64, 1},  {64, 128, 2},
+                                {128, 128, 2}, {128, 256, 3}, {256, 256, 3}, {384, 128, 2}, {128, 128, 2},
+                                {192, 64, 1},  {64, 64, 1},   {96, 32, 0},   {32, 32, 0}};
+    long long off = 0;
+    for (int i = 0; i < N_CONV; ++i) {
+        ConvL& L = n->conv[i];
+        L.cin = tab[i][0];
+        L.cout = tab[i][1];
+        L.level = tab[i][2];
+        L.relu = true;
+        L.w_off = off;
+        off += 27LL * L.cin * L.cout;
+        L.b_off = off;
+        off += L.cout;
+    }
+    n->head_w = off;
+    off += N_OUT * HEAD_IN;
+    n->head_b = off;
+    off += N_OUT;
+    n->nparam = off;
+    int rc = 0;
+    if ((rc = nalloc_t(n, &n->param, n->nparam)) || (rc = nalloc_t(n, &n->grad, n->nparam)) ||
+        (rc = nalloc_t(n, &n->dacc, 64)))
+        return rc;
+    if (n->train && ((rc = nalloc_t(n, &n->am, n->nparam)) || (rc = nalloc_t(n, &n->av, n->nparam)))) return rc;
+    for (int i = 0; i < N_CONV; ++i) {
+        ConvL& L = n->conv[i];
+        const long long nw = 27LL * L.cin * L.cout;
+        if ((rc = nalloc_t(n, &L.Wb, nw)) || (rc = nalloc_t(n, &L.Wt, nw))) return rc;
+    }
+    const int nb = n->nbmax;
+    auto act = [&](bf16** p, int level, int C) { return nalloc_t(n, p, (long long)nb * n->V(level) * C); };
+    if ((rc = act(&n->x0, 0, 32)) || (rc = act(&n->e1a, 0, 32)) || (rc = act(&n->e1, 0, 32)) ||
+        (rc = act(&n->p1, 1, 32)) || (rc = act(&n->e2a, 1, 64)) || (rc = act(&n->e2, 1, 64)) ||
+        (rc = act(&n->p2, 2, 64)) || (rc = act(&n->e3a, 2, 128)) || (rc = act(&n->e3, 2, 128)) ||
+        (rc = act(&n->p3, 3, 128)) || (rc = act(&n->b4a, 3, 256)) || (rc = act(&n->b4, 3, 256)) ||
+        (rc = act(&n->u3, 2, 384)) || (rc = act(&n->d3a, 2, 128)) || (rc = act(&n->d3, 2, 128)) ||
+        (rc = act(&n->u2, 1, 192)) || (rc = act(&n->d2a, 1, 64)) || (rc = act(&n->d2, 1, 64)) ||
+        (rc = act(&n->u1, 0, 96)) || (rc = act(&n->d1a, 0, 32)) || (rc = act(&n->d1, 0, 32)))
+        return rc;
+    if ((rc = nalloc_t(n, &n->out, 4LL * nb * n->V(0)))) return rc;
+    if (n->train) {
+        if ((rc = nalloc_t(n, &n->tgt, 4LL * nb * n->V(0))) || (rc = act(&n->g_d1, 0, 32)) ||
+            (rc = act(&n->g_d1a, 0, 32)) || (rc = act(&n->g_u1, 0, 96)) || (rc = act(&n->g_d2, 1, 64)) ||
+            (rc = act(&n->g_d2a, 1, 64)) || (rc = act(&n->g_u2, 1, 192)) || (rc = act(&n->g_d3, 2, 128)) ||
+            (rc = act(&n->g_d3a, 2, 128)) || (rc = act(&n->g_u3, 2, 384)) || (rc = act(&n->g_b4, 3, 256)) ||
+            (rc = act(&n->g_b4a, 3, 256)) || (rc = act(&n->g_p3, 3, 128)) || (rc = act(&n->g_e3, 2, 128)) ||
+            (rc = act(&n->g_e3a, 2, 128)) || (rc = act(&n->g_p2, 2, 64)) || (rc = act(&n->g_e2, 1, 64)) ||
+            (rc = act(&n->g_e2a, 1, 64)) || (rc = act(&n->g_p1, 1, 32)) || (rc = act(&n->g_e1, 0, 32)) ||
+            (rc = act(&n->g_e1a, 0, 32)))
+            return rc;
+    }
+    // activation and gradient wiring of each convolution
+    ConvL* c = n->conv;
+    auto wire = [](ConvL& L, bf16* x, bf16* y, bf16* gx, bf16* gy) {
+        L.x = x;
+        L.y = y;
+        L.gx = gx;
+        L.gy = gy;
+    };
+    wire(c[0], n->x0, n->e1a, nullptr, n->g_e1a);
+    wire(c[1], n->e1a, n->e1, n->g_e1a, n->g_e1);
+    wire(c[2], n->p1, n->e2a, n->g_p1, n->g_e2a);
+    wire(c[3], n->e2a, n->e2, n->g_e2a, n->g_e2);
+    wire(c[4], n->p2, n->e3a, n->g_p2, n->g_e3a);
+    wire(c[5], n->e3a, n->e3, n->g_e3a, n->g_e3);
+    wire(c[6], n->p3, n->b4a, n->g_p3, n->g_b4a);
+    wire(c[7], n->b4a, n->b4, n->g_b4a, n->g_b4);
+    wire(c[8], n->u3, n->d3a, n->g_u3, n->g_d3a);
+    wire(c[9], n->d3a, n->d3, n->g_d3a, n->g_d3);
+    wire(c[10], n->u2, n->d2a, n->g_u2, n->g_d2a);
+    wire(c[11], n->d2a, n->d2, n->g_d2a, n->g_d2);
+    wire(c[12], n->u1, n->d1a, n->g_u1, n->g_d1a);
+    wire(c[13], n->d1a, n->d1, n->g_d1a, n->g_d1);
+    for (int i = 0; i < N_IN; ++i) {
+        n->nm.mean[i] = 0.f;
+        n->nm.inv[i] = 1.f;
+    }
+    for (int i = 0; i < N_OUT; ++i) n->nm.out_scale[i] = n->nm.out_inv[i] = 1.f;
+    return NICR_OK;
+}
+
+void cast_weights(Net* n) {
+    for (int i = 0; i < N_CONV; ++i) {
+        ConvL& L = n->conv[i];
+        const long long nw = 27LL * L.cin * L.cout;
+        hipLaunchKernelGGL(k_cast_w, dim3(nblk(nw)), dim3(256), 0, n->stream, 27, L.cout, L.cin, n->param + L.w_off,
+                           L.Wb, L.Wt);
+    }
+}
+
+// He-normal weights (fan-in 27 x input channels that carry data), zero biases;
+// the head starts at zero so the first prediction is "no change".
+int init_params(Net* n, uint64_t seed) {
+    std::vector<float> h((size_t)n->nparam, 0.f);
+    std::mt19937_64 rng(seed);
+    for (int i = 0; i < N_CONV; ++i) {
+        const ConvL& L = n->conv[i];
+        const int cin_real = i == 0 ? N_IN : L.cin;
+        std::normal_distribution<float> nd(0.f, std::sqrt(2.f / (27.f * cin_real)));
+        for (int t = 0; t < 27; ++t)
+            for (int co = 0; co < L.cout; ++co)
+                for (int ci = 0; ci < L.cin; ++ci)
+                    h[(size_t)(L.w_off + ((long long)t * L.cout + co) * L.cin + ci)] = ci < cin_real ? nd(rng) : 0.f;
+    }
+    HIPCHK(hipMemcpyAsync(n->param, h.data(), h.size() * 4, hipMemcpyHostToDevice, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    cast_weights(n);
+    return NICR_OK;
+}
+
+void forward(Net* n, int nb) {
+    hipStream_t s = n->stream;
+    ConvL* c = n->conv;
+    auto conv = [&](int i) {
+        const ConvL& L = c[i];
+        launch_conv(n, geo(n, nb, L.level, L.cin, L.cout, 27), L.x, L.Wb, n->param + L.b_off, L.y, L.relu);
+    };
+    auto pool = [&](int level, int C, const bf16* X, bf16* Y) {
+        const long long nq = (long long)nb * n->V(level + 1);
+        hipLaunchKernelGGL(k_pool, dim3(nblk(nq * (C / 8))), dim3(256), 0, s, (int)nq, n->S[0] >> level,
+                           n->S[1] >> level, n->S[2] >> level, C, X, Y);
+    };
+    auto upcat = [&](int level, int Cu, int Cs, const bf16* L, const bf16* Sk, bf16* U) {
+        const long long np = (long long)nb * n->V(level);
+        hipLaunchKernelGGL(k_up_cat, dim3(nblk(np * ((Cu + Cs) / 8))), dim3(256), 0, s, (int)np, n->S[0] >> level,
+                           n->S[1] >> level, n->S[2] >> level, Cu, Cs, L, Sk, U);
+    };
+    if (n->timing) hipEventRecord(n->ev[0], s);
+    conv(0);
+    conv(1);
+    pool(0, 32, n->e1, n->p1);
+    conv(2);
+    conv(3);
+    pool(1, 64, n->e2, n->p2);
+    conv(4);
+    conv(5);
+    pool(2, 128, n->e3, n->p3);
+    conv(6);
+    conv(7);
+    upcat(2, 256, 128, n->b4, n->e3, n->u3);
+    conv(8);
+    conv(9);
+    upcat(1, 128, 64, n->d3, n->e2, n->u2);
+    conv(10);
+    conv(11);
+    upcat(0, 64, 32, n->d2, n->e1, n->u1);
+    conv(12);
+    conv(13);
+    const long long P0 = (long long)nb * n->V(0);
+    hipLaunchKernelGGL(k_head, dim3(nblk(P0)), dim3(256), 0, s, (int)P0, n->d1, n->param + n->head_w,
+                       n->param + n->head_b, n->out);
+    if (n->timing) hipEventRecord(n->ev[1], s);
+}
+
+// Gradients of L = (1/3) sum (out - tgt)^2 into n->grad (accumulated; the
+// caller zeroes them); the loss sum goes to *loss_slot.
+void backward(Net* n, int nb, float* loss_slot) {
+    hipStream_t s = n->stream;
+    ConvL* c = n->conv;
+    const long long P0 = (long long)nb * n->V(0);
+    hipLaunchKernelGGL(k_head_bwd, dim3((unsigned)std::min<long long>(1024, nblk(P0))), dim3(HB_THREADS), 0, s,
+                       (int)P0, n->d1, n->out, n->tgt, n->param + n->head_w, n->g_d1, n->grad + n->head_w,
+                       n->grad + n->head_b, loss_slot);
+    // conv i backward: weight and bias gradients from gy (pre-activation
+    // gradient) and x; data gradient gx = conv(gy, Wt) (masked by the caller's
+    // ReLU where x is a ReLU output of a single consumer).
+    auto conv_bwd = [&](int i, bool data, bool mask_x) {
+        const ConvL& L = c[i];
+        const Geo g = geo(n, nb, L.level, L.cin, L.cout, 27);
+        launch_wgrad(n, g, L.x, L.gy, n->grad + L.w_off);
+        launch_bias_grad(n, g.positions(), L.cout, L.gy, n->grad + L.b_off);
+        if (!data) return;
+        const Geo gt = geo(n, nb, L.level, L.cout, L.cin, 27);
+        launch_conv(n, gt, L.gy, L.Wt, nullptr, L.gx, false);
+        if (mask_x) {
+            const long long n8 = (long long)g.positions() * L.cin / 8;
+            hipLaunchKernelGGL(k_relu_mask, dim3(nblk(n8)), dim3(256), 0, s, n8, L.x, L.gx);
+        }
+    };
+    auto up_bwd = [&](int level, int Cu, int Ct, const bf16* dU, const bf16* Lo, bf16* dL) {
+        const long long nq = (long long)nb * n->V(level + 1);
+        hipLaunchKernelGGL(k_up_bwd, dim3(nblk(nq * (Cu / 8))), dim3(256), 0, s, (int)nq, n->S[0] >> level,
+                           n->S[1] >> level, n->S[2] >> level, Cu, Ct, dU, Lo, dL);
+    };
+    auto pool_skip_bwd = [&](int level, int C, int Cu, int Ct, const bf16* dP, const bf16* dU, const bf16* E,
+                             bf16* dE) {
+        const long long np = (long long)nb * n->V(level);
+        hipLaunchKernelGGL(k_pool_skip_bwd, dim3(nblk(np * (C / 8))), dim3(256), 0, s, (int)np, n->S[0] >> level,
+                           n->S[1] >> level, n->S[2] >> level, C, Cu, Ct, dP, dU, E, dE);
+    };
+    conv_bwd(13, true, true);                        // -> g_d1a
+    conv_bwd(12, true, false);                       // -> g_u1 [P0][96]
+    up_bwd(0, 64, 96, n->g_u1, n->d2, n->g_d2);      // -> g_d2
+    conv_bwd(11, true, true);
+    conv_bwd(10, true, false);                       // -> g_u2 [P1][192]
+    up_bwd(1, 128, 192, n->g_u2, n->d3, n->g_d3);
+    conv_bwd(9, true, true);
+    conv_bwd(8, true, false);                        // -> g_u3 [P2][384]
+    up_bwd(2, 256, 384, n->g_u3, n->b4, n->g_b4);
+    conv_bwd(7, true, true);
+    conv_bwd(6, true, false);                        // -> g_p3
+    pool_skip_bwd(2, 128, 256, 384, n->g_p3, n->g_u3, n->e3, n->g_e3);
+    conv_bwd(5, true, true);
+    conv_bwd(4, true, false);                        // -> g_p2
+    pool_skip_bwd(1, 64, 128, 192, n->g_p2, n->g_u2, n->e2, n->g_e2);
+    conv_bwd(3, true, true);
+    conv_bwd(2, true, false);                        // -> g_p1
+    pool_skip_bwd(0, 32, 64, 96, n->g_p1, n->g_u1, n->e1, n->g_e1);
+    conv_bwd(1, true, true);
+    conv_bwd(0, false, false);
+}
+
+void adam_step(Net* n, float lr) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    ++n->adam_t;
+    const float c1 = 1.f / (1.f - std::pow(b1, (float)n->adam_t));
+    const float c2 = 1.f / (1.f - std::pow(b2, (float)n->adam_t));
+    hipLaunchKernelGGL(k_adam, dim3(nblk(n->nparam)), dim3(256), 0, n->stream, n->nparam, n->param, n->grad, n->am,
+                       n->av, lr, b1, b2, c1, c2, eps);
+    cast_weights(n);
+}
+
+// network input and target of batch slot `slot` from training sample `si`
+void load_sample(Net* n, int slot, int si, bool target) {
+    const long long V = n->V(0);
+    const float* in = n->ds_in + (long long)si * N_IN * V;
+    hipLaunchKernelGGL((k_input<float, float>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, in, in + V, in + 2 * V,
+                       in + 3 * V, n->nm, n->x0 + (long long)slot * V * 32);
+    if (target)
+        hipLaunchKernelGGL(k_target, dim3(nblk(V)), dim3(256), 0, n->stream, (int)V,
+                           n->ds_d + (long long)si * N_OUT * V, n->nm, n->tgt + 4LL * slot * V);
+}
+
+int check_ctx_shape(Net* n, Ctx* c) {
+    if (c->dim != 3 || c->P.nx != n->S[0] || c->P.ny != n->S[1] || c->P.nz != n->S[2])
+        return fail(NICR_E_INVALID, "surrogate: model mesh differs from the network's volume");
+    if (c->prec != PREC_FP64) return fail(NICR_E_INVALID, "surrogate: the model must step in binary64");
+    return NICR_OK;
+}
+
+// network input from the model's current state (binary64 O, Cr, phi and the mask)
+int input_from_ctx(Net* n, Ctx* c, int slot) {
+    if (c->mask_dirty) {
+        do_mask(c);
+        c->mask_dirty = false;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const long long V = n->V(0);
+    const int cur = c->cur;
+    bf16* x0 = n->x0 + (long long)slot * V * 32;
+    if (c->te == 8)
+        hipLaunchKernelGGL((k_input<double, double>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, c->o[cur],
+                           c->cr[cur], c->p[cur], (const double*)c->gb[mask_index(c)], n->nm, x0);
+    else
+        hipLaunchKernelGGL((k_input<double, float>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, c->o[cur],
+                           c->cr[cur], c->p[cur], (const float*)c->gb[mask_index(c)], n->nm, x0);
+    return NICR_OK;
+}
+
+}  // namespace nn
+}  // namespace nicr
+
+using namespace nicr::nn;
+
+extern "C" {
+
+int nicr_unet_create(int s0, int s1, int s2, int max_batch, int train, int max_samples, uint64_t seed, void** net) {
+    if (!net) return fail(NICR_E_INVALID, "nicr_unet_create: null handle");
+    *net = nullptr;
+    if (s0 <= 0 || s1 <= 0 || s2 <= 0 || s0 % 8 || s1 % 8 || s2 % 8)
+        return fail(NICR_E_INVALID, "nicr_unet_create: volume sides must be positive multiples of 8");
+    if (max_batch < 1 || max_samples < 0) return fail(NICR_E_INVALID, "nicr_unet_create: bad batch or sample count");
+    const long long V = (long long)s0 * s1 * s2;
+    if (V * max_batch * 384 >= (1LL << 40) || V * max_batch >= (1LL << 31) / 4)
+        return fail(NICR_E_INVALID, "nicr_unet_create: volume x batch too large");
+    Net* n = new Net();
+    n->S[0] = s0;
+    n->S[1] = s1;
+    n->S[2] = s2;
+    n->nbmax = max_batch;
+    n->train = train != 0;
+    int rc = 0;
+    hipError_t e = hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) rc = fail(NICR_E_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    if (!rc && (hipEventCreate(&n->ev[0]) != hipSuccess || hipEventCreate(&n->ev[1]) != hipSuccess))
+        rc = fail(NICR_E_DEVICE, "hipEventCreate failed");
+    if (!rc) rc = net_build(n);
+    if (!rc && max_samples > 0) {
+        n->ds_cap = max_samples;
+        if ((rc = nalloc_t(n, &n->ds_in, (long long)max_samples * N_IN * V)) == 0)
+            rc = nalloc_t(n, &n->ds_d, (long long)max_samples * N_OUT * V);
+    }
+    if (!rc) rc = init_params(n, seed);
+    if (rc) {
+        net_free(n);
+        return rc;
+    }
+    *net = n;
+    return NICR_OK;
+}
+
+int nicr_unet_destroy(void* h) {
+    if (h) net_free(static_cast<Net*>(h));
+    return NICR_OK;
+}
+
+int nicr_unet_param_count(void* h, long long* count) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || !count) return fail(NICR_E_INVALID, "null argument");
+    *count = n->nparam;
+    return NICR_OK;
+}
+
+int nicr_unet_device_bytes(void* h, size_t* bytes) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || !bytes) return fail(NICR_E_INVALID, "null argument");
+    *bytes = n->bytes;
+    return NICR_OK;
+}
+
+int nicr_unet_get_params(void* h, float* host) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || !host) return fail(NICR_E_INVALID, "null argument");
+    HIPCHK(hipMemcpyAsync(host, n->param, n->nparam * 4, hipMemcpyDeviceToHost, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    return NICR_OK;
+}
+
+int nicr_unet_set_params(void* h, const float* host) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || !host) return fail(NICR_E_INVALID, "null argument");
+    HIPCHK(hipMemcpyAsync(n->param, host, n->nparam * 4, hipMemcpyHostToDevice, n->stream));
+    cast_weights(n);
+    HIPCHK(hipStreamSynchronize(n->stream));
+    n->adam_t = 0;
+    if (n->train) {
+        HIPCHK(hipMemsetAsync(n->am, 0, n->nparam * 4, n->stream));
+        HIPCHK(hipMemsetAsync(n->av, 0, n->nparam * 4, n->stream));
+        HIPCHK(hipStreamSynchronize(n->stream));
+    }
+    return NICR_OK;
+}
+
+int nicr_unet_get_grads(void* h, float* host) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || !host) return fail(NICR_E_INVALID, "null argument");
+    HIPCHK(hipMemcpyAsync(host, n->grad, n->nparam * 4, hipMemcpyDeviceToHost, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    return NICR_OK;
+}
+
+int nicr_unet_set_norm(void* h, const float in_mean[4], const float in_scale[4], const float out_scale[3]) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || !in_mean || !in_scale || !out_scale) return fail(NICR_E_INVALID, "null argument");
+    for (int i = 0; i < N_IN; ++i) {
+        if (!(in_scale[i] > 0.f)) return fail(NICR_E_INVALID, "nicr_unet_set_norm: input scales must be > 0");
+        n->nm.mean[i] = in_mean[i];
+        n->nm.inv[i] = 1.f / in_scale[i];
+    }
+    for (int i = 0; i < N_OUT; ++i) {
+        if (!(out_scale[i] >= 0.f)) return fail(NICR_E_INVALID, "nicr_unet_set_norm: output scales must be >= 0");
+        n->nm.out_scale[i] = out_scale[i];
+        n->nm.out_inv[i] = out_scale[i] > 0.f ? 1.f / out_scale[i] : 0.f;
+    }
+    return NICR_OK;
+}
+
+int nicr_unet_get_norm(void* h, float in_mean[4], float in_scale[4], float out_scale[3]) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || !in_mean || !in_scale || !out_scale) return fail(NICR_E_INVALID, "null argument");
+    for (int i = 0; i < N_IN; ++i) {
+        in_mean[i] = n->nm.mean[i];
+        in_scale[i] = 1.f / n->nm.inv[i];
+    }
+    for (int i = 0; i < N_OUT; ++i) out_scale[i] = n->nm.out_scale[i];
+    return NICR_OK;
+}
+
+int nicr_unet_forward(void* h, int nb, const float* in, float* out) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || !in || !out) return fail(NICR_E_INVALID, "null argument");
+    if (nb < 1 || nb > n->nbmax) return fail(NICR_E_INVALID, "nicr_unet_forward: batch exceeds the network's maximum");
+    const long long V = n->V(0);
+    float* tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, (size_t)V * N_IN * 4));
+    int rc = NICR_OK;
+    for (int b = 0; b < nb && !rc; ++b) {
+        if (hipMemcpyAsync(tmp, in + (size_t)b * N_IN * V, (size_t)V * N_IN * 4, hipMemcpyHostToDevice, n->stream) !=
+            hipSuccess)
+            rc = fail(NICR_E_DEVICE, "nicr_unet_forward: upload failed");
+        hipLaunchKernelGGL((k_input<float, float>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, tmp, tmp + V,
+                           tmp + 2 * V, tmp + 3 * V, n->nm, n->x0 + (long long)b * V * 32);
+        if (hipStreamSynchronize(n->stream) != hipSuccess) rc = fail(NICR_E_DEVICE, "nicr_unet_forward: input");
+    }
+    if (!rc) {
+        forward(n, nb);
+        std::vector<float> o4((size_t)4 * nb * V);
+        if (hipMemcpyAsync(o4.data(), n->out, o4.size() * 4, hipMemcpyDeviceToHost, n->stream) != hipSuccess ||
+            hipStreamSynchronize(n->stream) != hipSuccess)
+            rc = fail(NICR_E_DEVICE, "nicr_unet_forward: download failed");
+        for (int b = 0; b < nb && !rc; ++b)
+            for (int c = 0; c < N_OUT; ++c)
+                for (long long p = 0; p < V; ++p) out[((size_t)b * N_OUT + c) * V + p] = o4[4 * ((size_t)b * V + p) + c];
+    }
+    hipFree(tmp);
+    return rc;
+}
+
+int nicr_unet_loss_backward(void* h, int nb, const float* in, const float* target, double* loss) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || !in || !target || !loss) return fail(NICR_E_INVALID, "null argument");
+    if (!n->train) return fail(NICR_E_STATE, "nicr_unet_loss_backward: network created without training buffers");
+    if (nb < 1 || nb > n->nbmax) return fail(NICR_E_INVALID, "nicr_unet_loss_backward: bad batch");
+    std::vector<float> out((size_t)nb * N_OUT * n->V(0));
+    int rc = nicr_unet_forward(h, nb, in, out.data());
+    if (rc) return rc;
+    const long long V = n->V(0);
+    std::vector<float> t4((size_t)4 * nb * V, 0.f);
+    for (int b = 0; b < nb; ++b)
+        for (int c = 0; c < N_OUT; ++c)
+            for (long long p = 0; p < V; ++p) t4[4 * ((size_t)b * V + p) + c] = target[((size_t)b * N_OUT + c) * V + p];
+    HIPCHK(hipMemcpyAsync(n->tgt, t4.data(), t4.size() * 4, hipMemcpyHostToDevice, n->stream));
+    HIPCHK(hipMemsetAsync(n->grad, 0, n->nparam * 4, n->stream));
+    HIPCHK(hipMemsetAsync(n->dacc, 0, 64 * 8, n->stream));
+    backward(n, nb, reinterpret_cast<float*>(n->dacc));
+    float l = 0.f;
+    HIPCHK(hipMemcpyAsync(&l, n->dacc, 4, hipMemcpyDeviceToHost, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    *loss = l / (3.0 * (double)nb * V);
+    return NICR_OK;
+}
+
+int nicr_unet_set_timing(void* h, int on) {
+    Net* n = static_cast<Net*>(h);
+    if (!n) return fail(NICR_E_INVALID, "null argument");
+    n->timing = on != 0;
+    n->fwd_ms_sum = 0.;
+    n->fwd_count = 0;
+    return NICR_OK;
+}
+
+int nicr_unet_forward_ms(void* h, double* ms) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || !ms) return fail(NICR_E_INVALID, "null argument");
+    *ms = n->fwd_count ? n->fwd_ms_sum / n->fwd_count : 0.;
+    return NICR_OK;
+}
+
+// One convolution of the network's kernels on host arrays (float32 in, rounded
+// to BF16 on upload): op 0 forward y = act(conv(x, w) + b) [P][cout]; op 1 data
+// gradient dx = conv^T(dy, w) [P][cin]; op 2 weight gradient dw [27][cout][cin]
+// from x and dy. x [P][cin], dy [P][cout], w [27][cout][cin], b [cout]; P = nb
+// s0 s1 s2 positions in the network's cell order.
+int nicr_nn_conv_check(int op, int nb, int s0, int s1, int s2, int cin, int cout, int relu, const float* x,
+                       const float* w, const float* b, const float* dy, float* out) {
+    if (op < 0 || op > 2 || nb < 1 || s0 < 1 || s1 < 1 || s2 < 1 || cin % 32 || cout % 32 || cin < 32 || cout < 32 ||
+        !w || !out)
+        return fail(NICR_E_INVALID, "nicr_nn_conv_check: bad argument");
+    if ((op != 1 && !x) || (op != 0 && !dy) || (op == 0 && !b))
+        return fail(NICR_E_INVALID, "nicr_nn_conv_check: missing operand");
+    const long long P = (long long)nb * s0 * s1 * s2;
+    if (P * std::max(cin, cout) >= (1LL << 31)) return fail(NICR_E_INVALID, "nicr_nn_conv_check: too large");
+    Net tmp;
+    Net* n = &tmp;
+    HIPCHK(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
+    n->S[0] = s0;
+    n->S[1] = s1;
+    n->S[2] = s2;
+    const long long nw = 27LL * cin * cout;
+    auto to_bf = [](const float* src, long long cnt) {
+        std::vector<bf16> v((size_t)cnt);
+        for (long long i = 0; i < cnt; ++i) v[(size_t)i] = (bf16)src[i];
+        return v;
+    };
+    bf16 *dx = nullptr, *dyb = nullptr, *wb = nullptr, *wt = nullptr, *yb = nullptr;
+    float *wf = nullptr, *bf = nullptr, *dwf = nullptr;
+    int rc = 0;
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        if (!rc && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, n->stream) != hipSuccess)
+            rc = fail(NICR_E_DEVICE, "nicr_nn_conv_check: upload");
+    };
+    if ((rc = nalloc_t(n, &wf, nw)) || (rc = nalloc_t(n, &wb, nw)) || (rc = nalloc_t(n, &wt, nw)) ||
+        (rc = nalloc_t(n, &bf, cout)) || (rc = nalloc_t(n, &dx, P * cin)) || (rc = nalloc_t(n, &dyb, P * cout)) ||
+        (rc = nalloc_t(n, &yb, P * cout)) || (rc = nalloc_t(n, &dwf, nw))) {
+    } else {
+        up(wf, w, nw * 4);
+        if (!rc)
+            hipLaunchKernelGGL(k_cast_w, dim3(nblk(nw)), dim3(256), 0, n->stream, 27, cout, cin, wf, wb, wt);
+        std::vector<bf16> hx, hdy;
+        if (x) {
+            hx = to_bf(x, P * cin);
+            up(dx, hx.data(), hx.size() * 2);
+        }
+        if (dy) {
+            hdy = to_bf(dy, P * cout);
+            up(dyb, hdy.data(), hdy.size() * 2);
+        }
+        if (b) up(bf, b, cout * 4);
+        std::vector<bf16> hb;
+        if (!rc) {
+            if (op == 0) {
+                launch_conv(n, geo(n, nb, 0, cin, cout, 27), dx, wb, bf, yb, relu != 0);
+                hb.resize((size_t)(P * cout));
+                if (hipMemcpyAsync(hb.data(), yb, hb.size() * 2, hipMemcpyDeviceToHost, n->stream) != hipSuccess)
+                    rc = fail(NICR_E_DEVICE, "download");
+            } else if (op == 1) {
+                launch_conv(n, geo(n, nb, 0, cout, cin, 27), dyb, wt, nullptr, dx, false);
+                hb.resize((size_t)(P * cin));
+                if (hipMemcpyAsync(hb.data(), dx, hb.size() * 2, hipMemcpyDeviceToHost, n->stream) != hipSuccess)
+                    rc = fail(NICR_E_DEVICE, "download");
+            } else {
+                if (hipMemsetAsync(dwf, 0, nw * 4, n->stream) != hipSuccess) rc = fail(NICR_E_DEVICE, "memset");
+                launch_wgrad(n, geo(n, nb, 0, cin, cout, 27), dx, dyb, dwf);
+                if (hipMemcpyAsync(out, dwf, nw * 4, hipMemcpyDeviceToHost, n->stream) != hipSuccess)
+                    rc = fail(NICR_E_DEVICE, "download");
+            }
+        }
+        if (!rc && hipStreamSynchronize(n->stream) != hipSuccess) rc = fail(NICR_E_DEVICE, "nicr_nn_conv_check: run");
+        if (!rc && hipGetLastError() != hipSuccess) rc = fail(NICR_E_DEVICE, "nicr_nn_conv_check: launch");
+        for (size_t i = 0; !rc && op != 2 && i < hb.size(); ++i) out[i] = (float)hb[i];
+    }
+    for (void* p : n->allocs) hipFree(p);
+    n->allocs.clear();
+    hipStreamDestroy(n->stream);
+    n->stream = nullptr;
+    return rc;
+}
+
+static int sur_buffers(Net* n);
+
+// ---- training set ----
+
+int nicr_sur_record(void* h, void* model, int jump, nicr_step_report* rep) {
+    Net* n = static_cast<Net*>(h);
+    Ctx* c = static_cast<Ctx*>(model);
+    if (!n || !c || jump < 1) return fail(NICR_E_INVALID, "nicr_sur_record: bad argument");
+    int rc = check_ctx_shape(n, c);
+    if (rc) return rc;
+    if (n->ds_n >= n->ds_cap) return fail(NICR_E_STATE, "nicr_sur_record: training set is full");
+    if (c->mask_dirty) {
+        do_mask(c);
+        c->mask_dirty = false;
+    }
+    const long long V = n->V(0);
+    if ((rc = sur_buffers(n))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int cur = c->cur;
+    float* in = n->ds_in + (long long)n->ds_n * N_IN * V;
+    hipLaunchKernelGGL(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], in);
+    hipLaunchKernelGGL(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[cur], in + V);
+    hipLaunchKernelGGL(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->p[cur], in + 2 * V);
+    if (c->te == 8)
+        hipLaunchKernelGGL(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V,
+                           (const double*)c->gb[mask_index(c)], in + 3 * V);
+    else
+        HIPCHK(hipMemcpyAsync(in + 3 * V, c->gb[mask_index(c)], V * 4, hipMemcpyDeviceToDevice, n->stream));
+    HIPCHK(hipMemcpyAsync(n->bk[0], c->o[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
+    HIPCHK(hipMemcpyAsync(n->bk[1], c->cr[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
+    HIPCHK(hipMemcpyAsync(n->bk[2], c->p[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    nicr_step_report r{};
+    if ((rc = nicr_step(c, jump, &r))) return rc;
+    if (rep) *rep = r;
+    if (r.status != NICR_STEP_OK) return NICR_OK;   // not recorded
+    const int nc = c->cur;
+    float* d = n->ds_d + (long long)n->ds_n * N_OUT * V;
+    hipLaunchKernelGGL(k_delta_float, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[nc], n->bk[0], d);
+    hipLaunchKernelGGL(k_delta_float, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[nc], n->bk[1], d + V);
+    hipLaunchKernelGGL(k_delta_float, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->p[nc], n->bk[2], d + 2 * V);
+    HIPCHK(hipStreamSynchronize(n->stream));
+    ++n->ds_n;
+    return NICR_OK;
+}
+
+int nicr_sur_samples(void* h, int* count) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || !count) return fail(NICR_E_INVALID, "null argument");
+    *count = n->ds_n;
+    return NICR_OK;
+}
+
+// Normalisation from the training set: input channel mean and standard
+// deviation, output channel RMS of the deltas (0 for a channel that never changes).
+int nicr_sur_fit_norm(void* h) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || n->ds_n < 1) return fail(NICR_E_STATE, "nicr_sur_fit_norm: empty training set");
+    const long long V = n->V(0);
+    HIPCHK(hipMemsetAsync(n->dacc, 0, 64 * 8, n->stream));
+    for (int s = 0; s < n->ds_n; ++s) {
+        for (int c = 0; c < N_IN; ++c)
+            hipLaunchKernelGGL(k_moments, dim3(256), dim3(256), 0, n->stream, V,
+                               n->ds_in + ((long long)s * N_IN + c) * V, n->dacc + 2 * c);
+        for (int c = 0; c < N_OUT; ++c)
+            hipLaunchKernelGGL(k_moments, dim3(256), dim3(256), 0, n->stream, V,
+                               n->ds_d + ((long long)s * N_OUT + c) * V, n->dacc + 2 * (N_IN + c));
+    }
+    double m[2 * (N_IN + N_OUT)];
+    HIPCHK(hipMemcpyAsync(m, n->dacc, sizeof(m), hipMemcpyDeviceToHost, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    const double cnt = (double)V * n->ds_n;
+    float mean[N_IN], sc[N_IN], os[N_OUT];
+    for (int c = 0; c < N_IN; ++c) {
+        const double mu = m[2 * c] / cnt, var = std::max(0., m[2 * c + 1] / cnt - mu * mu);
+        mean[c] = (float)mu;
+        sc[c] = var > 1e-30 ? (float)std::sqrt(var) : 1.f;
+    }
+    for (int c = 0; c < N_OUT; ++c) {
+        const double rms = std::sqrt(m[2 * (N_IN + c) + 1] / cnt);
+        os[c] = rms > 1e-300 ? (float)rms : 0.f;
+    }
+    return nicr_unet_set_norm(h, mean, sc, os);
+}
+
+// `iters` Adam iterations on random batches of `batch` training samples. The
+// per-iteration losses (mean squared error of the normalised deltas) go to
+// losses[iters] when given.
+int nicr_sur_train(void* h, int iters, int batch, float lr, uint64_t seed, float* losses) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || iters < 0) return fail(NICR_E_INVALID, "nicr_sur_train: bad argument");
+    if (!n->train) return fail(NICR_E_STATE, "nicr_sur_train: network created without training buffers");
+    if (n->ds_n < 1) return fail(NICR_E_STATE, "nicr_sur_train: empty training set");
+    if (batch < 1 || batch > n->nbmax) return fail(NICR_E_INVALID, "nicr_sur_train: bad batch");
+    if (iters > n->loss_cap) {
+        float* p = nullptr;
+        int rc = nalloc_t(n, &p, iters);
+        if (rc) return rc;
+        n->loss_acc = p;
+        n->loss_cap = iters;
+    }
+    if (iters == 0) return NICR_OK;
+    HIPCHK(hipMemsetAsync(n->loss_acc, 0, (size_t)iters * 4, n->stream));
+    std::mt19937_64 rng(seed);
+    std::uniform_int_distribution<int> pick(0, n->ds_n - 1);
+    for (int it = 0; it < iters; ++it) {
+        for (int b = 0; b < batch; ++b) load_sample(n, b, pick(rng), true);
+        forward(n, batch);
+        HIPCHK(hipMemsetAsync(n->grad, 0, n->nparam * 4, n->stream));
+        backward(n, batch, n->loss_acc + it);
+        adam_step(n, lr);
+    }
+    HIPCHK(hipGetLastError());
+    if (losses) {
+        HIPCHK(hipMemcpyAsync(losses, n->loss_acc, (size_t)iters * 4, hipMemcpyDeviceToHost, n->stream));
+        HIPCHK(hipStreamSynchronize(n->stream));
+        const double denom = 3.0 * (double)batch * n->V(0);
+        for (int i = 0; i < iters; ++i) losses[i] = (float)(losses[i] / denom);
+    }
+    HIPCHK(hipStreamSynchronize(n->stream));
+    return NICR_OK;
+}
+
+// Mean squared error of the normalised deltas over training samples [first,
+// first + count) (no parameter update).
+int nicr_sur_eval(void* h, int first, int count, double* mse) {
+    Net* n = static_cast<Net*>(h);
+    if (!n || !mse || first < 0 || count < 1 || first + count > n->ds_n)
+        return fail(NICR_E_INVALID, "nicr_sur_eval: bad sample range");
+    if (!n->train) return fail(NICR_E_STATE, "nicr_sur_eval: network created without training buffers");
+    HIPCHK(hipMemsetAsync(n->dacc, 0, 64 * 8, n->stream));
+    for (int s = first; s < first + count; ++s) {
+        load_sample(n, 0, s, true);
+        forward(n, 1);
+        HIPCHK(hipMemsetAsync(n->grad, 0, n->nparam * 4, n->stream));
+        backward(n, 1, reinterpret_cast<float*>(n->dacc));
+    }
+    float l = 0.f;
+    HIPCHK(hipMemcpyAsync(&l, n->dacc, 4, hipMemcpyDeviceToHost, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    *mse = l / (3.0 * (double)count * n->V(0));
+    return NICR_OK;
+}
+
+// ---- surrogate-accelerated stepping ----
+
+static int sur_sum(Ctx* c, int field, double* s) {
+    nicr_stats st;
+    int rc = nicr_field_stats(c, field, &st);
+    if (rc) return rc;
+    *s = st.sum[0] + st.sum[1];
+    return NICR_OK;
+}
+
+static int sur_buffers(Net* n) {
+    if (n->bk[0]) return NICR_OK;
+    const long long V = n->V(0);
+    int rc = 0;
+    for (int i = 0; i < 3; ++i)
+        if ((rc = nalloc_t(n, &n->bk[i], V))) return rc;
+    for (int i = 0; i < 2; ++i)
+        if ((rc = nalloc_t(n, &n->du[i], V))) return rc;
+    n->bk_cells = V;
+    return NICR_OK;
+}
+
+// ||R|| / ||xh - x0|| of the trapezoid residual (infinite for a zero jump)
+static int sur_trap(Net* n, Ctx* c, const double* xh, const double* xh1, const double* x0, const double* u0, int J,
+                    double* ratio) {
+    double acc[2];
+    HIPCHK(hipMemsetAsync(n->dacc, 0, 2 * 8, n->stream));
+    hipLaunchKernelGGL(k_trap, dim3(1024), dim3(256), 0, n->stream, c->ncell, xh, xh1, x0, u0, 0.5 * J, n->dacc);
+    HIPCHK(hipMemcpyAsync(acc, n->dacc, 2 * 8, hipMemcpyDeviceToHost, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    *ratio = acc[1] > 0. ? std::sqrt(acc[0] / acc[1]) : INFINITY;
+    return NICR_OK;
+}
+
+int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* opt, nicr_sur_report* rep) {
+    Net* n = static_cast<Net*>(h);
+    Ctx* c = static_cast<Ctx*>(model);
+    if (!n || !c || !opt || !rep || nsteps < 0) return fail(NICR_E_INVALID, "nicr_sur_advance: bad argument");
+    int rc = check_ctx_shape(n, c);
+    if (rc) return rc;
+    const int J = opt->jump;
+    if (J < 2) return fail(NICR_E_INVALID, "nicr_sur_advance: jump must be >= 2");
+    if (!(opt->residual_tol > 0.)) return fail(NICR_E_INVALID, "nicr_sur_advance: residual_tol must be > 0");
+    if ((rc = sur_buffers(n))) return rc;
+    const long long V = n->V(0);
+    *rep = nicr_sur_report{};
+    const bool conserve = opt->conserve_mass != 0;
+    int left = nsteps;
+    nicr_step_report r{};
+    auto plain = [&](int k) -> int {
+        if (k <= 0) return NICR_OK;
+        int e = nicr_step(c, k, &r);
+        if (e) return e;
+        rep->plain_steps += r.steps_done;
+        left -= r.steps_done;
+        if (r.status != NICR_STEP_OK) {
+            rep->step = r;
+            rep->status = r.status;
+        }
+        return NICR_OK;
+    };
+    while (left > 0 && rep->status == NICR_STEP_OK) {
+        if (left < J + 2) {
+            if ((rc = plain(left))) return rc;
+            break;
+        }
+        // the plain step before the jump: its update u0 enters the residual
+        if ((rc = plain(1))) return rc;
+        if (rep->status != NICR_STEP_OK) break;
+        const int cur = c->cur;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], c->o[cur ^ 1], n->du[0]);
+        hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[cur], c->cr[cur ^ 1], n->du[1]);
+        HIPCHK(hipMemcpyAsync(n->bk[0], c->o[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
+        HIPCHK(hipMemcpyAsync(n->bk[1], c->cr[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
+        HIPCHK(hipMemcpyAsync(n->bk[2], c->p[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
+        double m0[2] = {0., 0.};
+        if (conserve && ((rc = sur_sum(c, NICR_FIELD_O, &m0[0])) || (rc = sur_sum(c, NICR_FIELD_CR, &m0[1]))))
+            return rc;
+        DevCtl saved;
+        if ((rc = sync_ctl(c, &saved))) return rc;
+        // proposal x0 + network(x0)
+        if ((rc = input_from_ctx(n, c, 0))) return rc;
+        forward(n, 1);
+        hipLaunchKernelGGL(k_apply_jump, dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, n->out, n->nm, c->o[cur],
+                           c->cr[cur], c->p[cur]);
+        HIPCHK(hipStreamSynchronize(n->stream));
+        if (n->timing) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, n->ev[0], n->ev[1]) == hipSuccess) {
+                n->fwd_ms_sum += ms;
+                ++n->fwd_count;
+            }
+        }
+        c->refresh_valid = false;
+        bool ok = true;
+        nicr_stats so, scr, sp;
+        if ((rc = nicr_field_stats(c, NICR_FIELD_O, &so)) || (rc = nicr_field_stats(c, NICR_FIELD_CR, &scr)) ||
+            (rc = nicr_field_stats(c, NICR_FIELD_PHI, &sp)))
+            return rc;
+        if (conserve) {
+            const double s_o = so.sum[0] + so.sum[1], s_c = scr.sum[0] + scr.sum[1];
+            if (s_o > 0. && m0[0] > 0.)
+                hipLaunchKernelGGL(k_scale, dim3(nblk(V)), dim3(256), 0, c->stream, V, c->o[cur], m0[0] / s_o);
+            if (s_c > 0. && m0[1] > 0.)
+                hipLaunchKernelGGL(k_scale, dim3(nblk(V)), dim3(256), 0, c->stream, V, c->cr[cur], m0[1] / s_c);
+            if ((rc = nicr_field_stats(c, NICR_FIELD_O, &so)) || (rc = nicr_field_stats(c, NICR_FIELD_CR, &scr)))
+                return rc;
+        }
+        const double lo = opt->min_value, hi = opt->max_value;
+        if (!(so.min >= lo && so.max <= hi && scr.min >= lo && scr.max <= hi && sp.min >= lo && sp.max <= hi))
+            ok = false;
+        double ro = INFINITY, rcr = INFINITY;
+        if (ok) {
+            nicr_step_report rr{};
+            if ((rc = nicr_step(c, 1, &rr))) return rc;
+            if (rr.status != NICR_STEP_OK) ok = false;
+        }
+        if (ok) {
+            const int k = c->cur;   // proposal in k ^ 1, one step after it in k
+            if ((rc = sur_trap(n, c, c->o[k ^ 1], c->o[k], n->bk[0], n->du[0], J, &ro)) ||
+                (rc = sur_trap(n, c, c->cr[k ^ 1], c->cr[k], n->bk[1], n->du[1], J, &rcr)))
+                return rc;
+            if (!(ro <= opt->residual_tol && rcr <= opt->residual_tol)) ok = false;
+        }
+        if (ok) {
+            ++rep->jumps_accepted;
+            rep->max_residual_o = std::max(rep->max_residual_o, ro);
+            rep->max_residual_cr = std::max(rep->max_residual_cr, rcr);
+            rep->plain_steps += 1;
+            left -= J + 1;
+            continue;
+        }
+        // reject: restore the control block (state index, budgets) and the
+        // fields before the jump (eta and the mask were not touched by the
+        // jump; a check step wrote only the other buffers), then take the
+        // plain steps
+        ++rep->jumps_rejected;
+        if (std::isfinite(ro) && std::isfinite(rcr))
+            rep->min_rejected_residual = rep->jumps_rejected == 1 ? std::max(ro, rcr)
+                                                                  : std::min(rep->min_rejected_residual, std::max(ro, rcr));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipMemcpyAsync(c->ctl, &saved, sizeof(saved), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->cur = saved.cur;
+        const int cb = c->cur;
+        HIPCHK(hipMemcpyAsync(c->o[cb], n->bk[0], V * 8, hipMemcpyDeviceToDevice, n->stream));
+        HIPCHK(hipMemcpyAsync(c->cr[cb], n->bk[1], V * 8, hipMemcpyDeviceToDevice, n->stream));
+        HIPCHK(hipMemcpyAsync(c->p[cb], n->bk[2], V * 8, hipMemcpyDeviceToDevice, n->stream));
+        HIPCHK(hipStreamSynchronize(n->stream));
+        c->refresh_valid = false;
+        if ((rc = plain(J))) return rc;
+    }
+    rep->steps_done = nsteps - left;
+    return NICR_OK;
+}
+
+}  // extern "C"

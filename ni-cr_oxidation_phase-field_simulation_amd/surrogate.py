@@ -1,0 +1,226 @@
+"""Learned surrogate of the step over the C ABI of include/nicr_nn.h (libnicr.so).
+
+`UNet` owns a device network (BF16 MFMA kernels, FP32 master weights) for
+volumes of the model's 3D mesh; `Surrogate` records training samples from a
+binary64 `Model`, trains the network on the GPU and advances a model with
+network jumps that the plain step accepts or rejects. No counterpart in the
+reference. There is no CPU path: every call runs the library's kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import NicrStepReport, load_library
+
+_VOIDP = ctypes.c_void_p
+_FPTR = ctypes.POINTER(ctypes.c_float)
+_DPTR = ctypes.POINTER(ctypes.c_double)
+
+
+class SurOptions(ctypes.Structure):
+    _fields_ = [("jump", ctypes.c_int32), ("conserve_mass", ctypes.c_int32), ("residual_tol", ctypes.c_double),
+                ("min_value", ctypes.c_double), ("max_value", ctypes.c_double)]
+
+
+class SurReport(ctypes.Structure):
+    _fields_ = [("steps_done", ctypes.c_int32), ("plain_steps", ctypes.c_int32), ("jumps_accepted", ctypes.c_int32),
+                ("jumps_rejected", ctypes.c_int32), ("status", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("max_residual_o", ctypes.c_double), ("max_residual_cr", ctypes.c_double),
+                ("min_rejected_residual", ctypes.c_double), ("step", NicrStepReport)]
+
+
+def _declare(lib):
+    if getattr(lib, "_nn_declared", False):
+        return
+    i, ll, u64, f, d = ctypes.c_int, ctypes.c_longlong, ctypes.c_uint64, ctypes.c_float, ctypes.c_double
+    sigs = {
+        "nicr_unet_create": [i, i, i, i, i, i, u64, ctypes.POINTER(_VOIDP)],
+        "nicr_unet_destroy": [_VOIDP],
+        "nicr_unet_param_count": [_VOIDP, ctypes.POINTER(ll)],
+        "nicr_unet_device_bytes": [_VOIDP, ctypes.POINTER(ctypes.c_size_t)],
+        "nicr_unet_get_params": [_VOIDP, _FPTR],
+        "nicr_unet_set_params": [_VOIDP, _FPTR],
+        "nicr_unet_get_grads": [_VOIDP, _FPTR],
+        "nicr_unet_set_norm": [_VOIDP, _FPTR, _FPTR, _FPTR],
+        "nicr_unet_get_norm": [_VOIDP, _FPTR, _FPTR, _FPTR],
+        "nicr_unet_forward": [_VOIDP, i, _FPTR, _FPTR],
+        "nicr_unet_loss_backward": [_VOIDP, i, _FPTR, _FPTR, _DPTR],
+        "nicr_unet_set_timing": [_VOIDP, i],
+        "nicr_unet_forward_ms": [_VOIDP, _DPTR],
+        "nicr_nn_conv_check": [i, i, i, i, i, i, i, i, _FPTR, _FPTR, _FPTR, _FPTR, _FPTR],
+        "nicr_sur_record": [_VOIDP, _VOIDP, i, ctypes.POINTER(NicrStepReport)],
+        "nicr_sur_samples": [_VOIDP, ctypes.POINTER(i)],
+        "nicr_sur_fit_norm": [_VOIDP],
+        "nicr_sur_train": [_VOIDP, i, i, f, u64, _FPTR],
+        "nicr_sur_eval": [_VOIDP, i, i, _DPTR],
+        "nicr_sur_advance": [_VOIDP, _VOIDP, i, ctypes.POINTER(SurOptions), ctypes.POINTER(SurReport)],
+    }
+    for name, args in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    lib._nn_declared = True
+
+
+def _lib():
+    lib = load_library()
+    _declare(lib)
+    return lib
+
+
+def _check(lib, rc):
+    if rc != 0:
+        raise RuntimeError(lib.nicr_last_error().decode())
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fp(a):
+    return a.ctypes.data_as(_FPTR)
+
+
+def conv_check(op: int, x, w, b, dy, relu: bool = False):
+    """One 3x3x3 convolution of the network's kernels on host arrays.
+
+    x: (nb, s0, s1, s2, cin), dy: (nb, s0, s1, s2, cout), w: (27, cout, cin),
+    b: (cout,). op 0 -> y (nb, s0, s1, s2, cout); op 1 -> dx (.., cin); op 2 -> dw (27, cout, cin)."""
+    lib = _lib()
+    w = _f32(w)
+    cout, cin = w.shape[1], w.shape[2]
+    ref = x if x is not None else dy
+    nb, s0, s1, s2 = ref.shape[:4]
+    xs = _f32(x) if x is not None else None
+    dys = _f32(dy) if dy is not None else None
+    bs = _f32(b) if b is not None else None
+    if op == 0:
+        out = np.empty((nb, s0, s1, s2, cout), np.float32)
+    elif op == 1:
+        out = np.empty((nb, s0, s1, s2, cin), np.float32)
+    else:
+        out = np.empty((27, cout, cin), np.float32)
+    _check(lib, lib.nicr_nn_conv_check(op, nb, s0, s1, s2, cin, cout, int(relu),
+                                       _fp(xs) if xs is not None else None, _fp(w),
+                                       _fp(bs) if bs is not None else None,
+                                       _fp(dys) if dys is not None else None, _fp(out)))
+    return out
+
+
+class UNet:
+    """Device U-Net for volumes `shape` = (s0, s1, s2), multiples of 8."""
+
+    def __init__(self, shape, max_batch: int = 1, train: bool = False, max_samples: int = 0, seed: int = 1):
+        lib = _lib()
+        self.lib = lib
+        self.shape = tuple(int(s) for s in shape)
+        h = _VOIDP()
+        _check(lib, lib.nicr_unet_create(*self.shape, int(max_batch), int(bool(train)), int(max_samples), int(seed),
+                                         ctypes.byref(h)))
+        self.h = h
+        n = ctypes.c_longlong(0)
+        _check(lib, lib.nicr_unet_param_count(h, ctypes.byref(n)))
+        self.nparam = n.value
+
+    @property
+    def cells(self) -> int:
+        return int(np.prod(self.shape))
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_size_t(0)
+        _check(self.lib, self.lib.nicr_unet_device_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparam, np.float32)
+        _check(self.lib, self.lib.nicr_unet_get_params(self.h, _fp(out)))
+        return out
+
+    def set_params(self, p) -> None:
+        p = _f32(p)
+        assert p.size == self.nparam
+        _check(self.lib, self.lib.nicr_unet_set_params(self.h, _fp(p)))
+
+    def grads(self) -> np.ndarray:
+        out = np.empty(self.nparam, np.float32)
+        _check(self.lib, self.lib.nicr_unet_get_grads(self.h, _fp(out)))
+        return out
+
+    def set_norm(self, in_mean, in_scale, out_scale) -> None:
+        a, b, c = _f32(in_mean), _f32(in_scale), _f32(out_scale)
+        _check(self.lib, self.lib.nicr_unet_set_norm(self.h, _fp(a), _fp(b), _fp(c)))
+
+    def norm(self):
+        a, b, c = np.empty(4, np.float32), np.empty(4, np.float32), np.empty(3, np.float32)
+        _check(self.lib, self.lib.nicr_unet_get_norm(self.h, _fp(a), _fp(b), _fp(c)))
+        return a, b, c
+
+    def forward(self, x):
+        """x: (nb, 4, s0, s1, s2) float32 -> (nb, 3, s0, s1, s2) normalised changes."""
+        x = _f32(x)
+        out = np.empty((x.shape[0], 3) + self.shape, np.float32)
+        _check(self.lib, self.lib.nicr_unet_forward(self.h, x.shape[0], _fp(x), _fp(out)))
+        return out
+
+    def loss_backward(self, x, target) -> float:
+        x, t = _f32(x), _f32(target)
+        loss = ctypes.c_double(0.0)
+        _check(self.lib, self.lib.nicr_unet_loss_backward(self.h, x.shape[0], _fp(x), _fp(t), ctypes.byref(loss)))
+        return loss.value
+
+    # ---- training set and training ----
+    def record(self, model, jump: int) -> NicrStepReport:
+        rep = NicrStepReport()
+        _check(self.lib, self.lib.nicr_sur_record(self.h, model._h.h, int(jump), ctypes.byref(rep)))
+        return rep
+
+    @property
+    def samples(self) -> int:
+        n = ctypes.c_int(0)
+        _check(self.lib, self.lib.nicr_sur_samples(self.h, ctypes.byref(n)))
+        return n.value
+
+    def fit_norm(self) -> None:
+        _check(self.lib, self.lib.nicr_sur_fit_norm(self.h))
+
+    def train(self, iters: int, batch: int = 1, lr: float = 1e-3, seed: int = 1) -> np.ndarray:
+        losses = np.zeros(max(1, iters), np.float32)
+        _check(self.lib, self.lib.nicr_sur_train(self.h, int(iters), int(batch), float(lr), int(seed), _fp(losses)))
+        return losses[:iters]
+
+    def eval(self, first: int, count: int) -> float:
+        mse = ctypes.c_double(0.0)
+        _check(self.lib, self.lib.nicr_sur_eval(self.h, int(first), int(count), ctypes.byref(mse)))
+        return mse.value
+
+    # ---- surrogate stepping ----
+    def advance(self, model, nsteps: int, jump: int = 50, conserve_mass: bool = True, residual_tol: float = 0.1,
+                bounds=(0.0, 1.0)) -> SurReport:
+        """Advance a binary64 Model by nsteps with accepted network jumps (nicr_sur_advance)."""
+        opt = SurOptions(int(jump), int(bool(conserve_mass)), float(residual_tol), float(bounds[0]),
+                         float(bounds[1]))
+        rep = SurReport()
+        _check(self.lib, self.lib.nicr_sur_advance(self.h, model._h.h, int(nsteps), ctypes.byref(opt),
+                                                   ctypes.byref(rep)))
+        return rep
+
+    def set_timing(self, on: bool) -> None:
+        _check(self.lib, self.lib.nicr_unet_set_timing(self.h, int(bool(on))))
+
+    def forward_ms(self) -> float:
+        ms = ctypes.c_double(0.0)
+        _check(self.lib, self.lib.nicr_unet_forward_ms(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.nicr_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -130,3 +130,18 @@ def eta_from_labels(labels, dist, colour, n_eta, ell=1.5, metal=None):
     if metal is not None:
         eta *= metal
     return eta
+
+
+def closed_alloy(shape, n_seeds=16, seed=1, cr_mean=0.20, cr_noise=0.005, o_background=1e-3, o_noise=0.2):
+    """State of the surrogate workloads: static Voronoi grains (GB mask of the
+    fixed-grain mode), Cr = cr_mean + U(-cr_noise, cr_noise), O = o_background
+    (1 + U(-o_noise, o_noise)), no oxide (phi = 0), no inlet layer. With closed
+    boundaries the step conserves O and Cr and no cell leaves [0, 1].
+    Returns (o, cr, p, gb)."""
+    rng = np.random.default_rng(seed)
+    _, dist, _ = voronoi_labels(shape, n_seeds, seed=seed)
+    gb = gb_mask_from_distance(dist)
+    cr = cr_mean + rng.uniform(-cr_noise, cr_noise, size=shape)
+    o = o_background * (1.0 + rng.uniform(-o_noise, o_noise, size=shape))
+    p = np.zeros(shape)
+    return o, cr, p, gb

@@ -1,0 +1,225 @@
+"""Learned surrogate: BF16 MFMA convolution kernels, U-Net forward and
+gradients against a CPU float32 torch reference, training on the GPU, and the
+accept/reject logic of surrogate jumps against the plain binary64 step.
+
+Bars:
+- one convolution (forward, data gradient): each output within 2^-8 relative
+  (+1e-5 of the largest) of torch's float32 result on the same bfloat16 inputs
+  -- the device rounds its float32 sums to bfloat16 once;
+- weight gradient: float32 sums of exact bfloat16 products, within 1e-4 of the
+  largest entry;
+- whole network (tests/unet_ref.py rounds to bfloat16 where the device stores
+  bfloat16): outputs within 1e-2 relative L2; every parameter block's gradient
+  within 5e-2 relative L2 and cosine > 0.999 (different float32 summation orders
+  flip single bfloat16 roundings; their effect grows along the backward chain to
+  about 2.5e-2 at the 2^3-cell bottleneck, while a wiring mistake -- a wrong skip
+  channel range, a missing ReLU mask -- costs O(1));
+- a surrogate whose every jump is rejected leaves the model bitwise equal to
+  plain stepping (fields and budgets): rejection restores the state exactly.
+"""
+import importlib
+
+import numpy as np
+import pytest
+import torch
+
+import unet_ref as R
+
+pytestmark = pytest.mark.gpu
+
+PKG = "ni-cr_oxidation_phase-field_simulation_amd"
+OUT_RTOL = 2 ** -8
+GRAD_REL_L2 = 5e-2
+GRAD_COS = 0.999
+
+
+def _sur():
+    return importlib.import_module(PKG + ".surrogate")
+
+
+def _syn():
+    return importlib.import_module(PKG + ".synthetic")
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32))
+
+
+CONV_CASES = [
+    (2, 4, 6, 5, 32, 32),     # partial block, odd sides: zero pad in X, wrap in Y and Z
+    (1, 6, 8, 8, 64, 64),     # 64-wide output tiles
+    (1, 4, 4, 8, 96, 32),     # concatenated input
+    (1, 2, 4, 4, 384, 128),
+    (2, 2, 2, 2, 256, 256),   # bottleneck of a 16^3 volume
+    (1, 3, 1, 1, 32, 64),     # a single Y/Z cell: every wrapped tap is the cell itself
+]
+
+
+def _conv_case(case, seed):
+    nb, s0, s1, s2, cin, cout = case
+    rng = np.random.default_rng(seed)
+    x = R.bf16(rng.normal(size=(nb, s0, s1, s2, cin)))
+    w = R.bf16(rng.normal(0, 1.0 / np.sqrt(27 * cin), size=(27, cout, cin)))
+    b = rng.normal(0, 0.1, size=cout).astype(np.float32)
+    dy = R.bf16(rng.normal(size=(nb, s0, s1, s2, cout)))
+    return x, w, b, dy
+
+
+def _close(a, ref, rtol=OUT_RTOL):
+    tol = rtol * np.abs(ref) + 1e-5 * np.abs(ref).max()
+    bad = np.abs(a - ref) > tol
+    assert not bad.any(), f"{bad.sum()} of {bad.size} entries off; max |d| {np.abs(a - ref).max():.3e}"
+
+
+@pytest.mark.parametrize("case", CONV_CASES)
+def test_conv_forward_vs_torch(case):
+    x, w, b, _ = _conv_case(case, 1)
+    y = _sur().conv_check(0, x, w, b, None, relu=True)
+    ref = torch.relu(R.conv3(_t(x).permute(0, 4, 1, 2, 3), _t(w), _t(b))).permute(0, 2, 3, 4, 1).numpy()
+    _close(y, ref)
+
+
+@pytest.mark.parametrize("case", CONV_CASES)
+def test_conv_data_gradient_vs_torch(case):
+    x, w, _, dy = _conv_case(case, 2)
+    dx = _sur().conv_check(1, None, w, None, dy)
+    xt = _t(x).permute(0, 4, 1, 2, 3).contiguous().requires_grad_(True)
+    y = R.conv3(xt, _t(w), None)
+    (g,) = torch.autograd.grad(y, xt, _t(dy).permute(0, 4, 1, 2, 3))
+    _close(dx, g.permute(0, 2, 3, 4, 1).numpy())
+
+
+@pytest.mark.parametrize("case", CONV_CASES)
+def test_conv_weight_gradient_vs_torch(case):
+    x, w, _, dy = _conv_case(case, 3)
+    dw = _sur().conv_check(2, x, w, None, dy)
+    wt = _t(w).requires_grad_(True)
+    y = R.conv3(_t(x).permute(0, 4, 1, 2, 3), wt, None)
+    (g,) = torch.autograd.grad(y, wt, _t(dy).permute(0, 4, 1, 2, 3))
+    g = g.numpy()
+    assert np.abs(dw - g).max() <= 1e-4 * np.abs(g).max()
+
+
+def _random_net_params(net, seed):
+    rng = np.random.default_rng(seed)
+    p = net.params()
+    convs, _ = R.split_params(p)
+    off = 0
+    for cin, cout in R.CONVS:
+        off += 27 * cin * cout
+        p[off:off + cout] = rng.normal(0, 0.05, size=cout)
+        off += cout
+    p[off:off + 99] = rng.normal(0, 0.3, size=99)
+    return p
+
+
+@pytest.mark.parametrize("shape,nb", [((16, 16, 16), 2), ((8, 16, 24), 1)])
+def test_unet_forward_and_gradients_vs_torch(shape, nb):
+    sur = _sur()
+    net = sur.UNet(shape, max_batch=nb, train=True, seed=7)
+    assert net.nparam == R.param_count()
+    p = _random_net_params(net, 3)
+    net.set_params(p)
+    rng = np.random.default_rng(5)
+    fields = np.stack([rng.uniform(0, 2e-3, size=(nb,) + shape), 0.2 + 0.01 * rng.normal(size=(nb,) + shape),
+                       rng.uniform(0, 1, size=(nb,) + shape), rng.uniform(0, 1, size=(nb,) + shape)], axis=1)
+    mean = np.array([1e-3, 0.2, 0.5, 0.5], np.float32)
+    scale = np.array([6e-4, 0.01, 0.3, 0.3], np.float32)
+    net.set_norm(mean, scale, np.ones(3, np.float32))
+    target = rng.normal(size=(nb, 3) + shape).astype(np.float32)
+    out = net.forward(fields)
+    ref_out, ref_loss, ref_g = R.loss_and_grads(net.params(), fields, target, mean, scale)
+    err = np.linalg.norm(out - ref_out) / np.linalg.norm(ref_out)
+    assert err < 1e-2, f"forward rel L2 {err:.3e}"
+    loss = net.loss_backward(fields, target)
+    assert abs(loss - ref_loss / np.prod(shape) / nb) <= 2e-2 * abs(loss)
+    g = net.grads()
+    off = 0
+    names = [f"conv{i}" for i in range(len(R.CONVS))]
+    blocks = []
+    for (cin, cout), nm in zip(R.CONVS, names):
+        blocks += [(nm + ".W", off, off + 27 * cin * cout), (nm + ".b", off + 27 * cin * cout,
+                                                             off + 27 * cin * cout + cout)]
+        off += 27 * cin * cout + cout
+    blocks += [("head.W", off, off + 96), ("head.b", off + 96, off + 99)]
+    for nm, a, b in blocks:
+        ga, gr = g[a:b], ref_g[a:b]
+        if np.linalg.norm(gr) == 0:
+            assert np.linalg.norm(ga) == 0, nm
+            continue
+        rel = np.linalg.norm(ga - gr) / np.linalg.norm(gr)
+        cos = float(np.dot(ga, gr) / (np.linalg.norm(ga) * np.linalg.norm(gr)))
+        assert rel < GRAD_REL_L2 and cos > GRAD_COS, f"{nm}: gradient rel L2 {rel:.3e}, cosine {cos:.6f}"
+    net.close()
+
+
+def _closed_model(nicr, shape, seed, **kw):
+    o, cr, p, gb = _syn().closed_alloy(shape, n_seeds=8, seed=seed)
+    cfg = nicr.make_config(3, nx=shape[0], ny=shape[1], nz=shape[2], grains=True, grain_evolution=False, bc="closed",
+                           ck="off", bounds="legacy", nucleation="off", **kw)
+    m = nicr.Model(cfg)
+    for k, a in zip(("o", "cr", "p", "gb"), (o, cr, p, gb)):
+        m.set(k, a)
+    return m
+
+
+def test_rejected_jumps_equal_plain_steps(nicr):
+    """An untrained network (zero head: the proposal does not move) is rejected
+    at every jump; the model then equals plain stepping bitwise."""
+    shape = (16, 16, 24)
+    a = _closed_model(nicr, shape, 4)
+    b = _closed_model(nicr, shape, 4)
+    net = _sur().UNet(shape, seed=1)
+    net.set_norm([1e-3, 0.2, 0.0, 0.5], [2e-4, 3e-3, 1.0, 0.3], [1e-6, 1e-5, 0.0])
+    rep = net.advance(a, 130, jump=20, residual_tol=0.2)
+    assert rep.status == 0 and rep.steps_done == 130
+    assert rep.jumps_accepted == 0 and rep.jumps_rejected >= 5
+    assert rep.plain_steps == 130
+    rb = b.step(130)
+    assert rb.steps_done == 130
+    for f in ("o", "cr", "p"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    assert a.b == b.b
+    net.close()
+
+
+def test_training_reduces_loss_and_jumps_are_checked(nicr):
+    """Samples from plain trajectories, Adam on the GPU: the training loss falls;
+    accepted jumps keep O and Cr totals and stay close to plain stepping."""
+    shape = (32, 32, 32)
+    net = _sur().UNet(shape, max_batch=2, train=True, max_samples=12, seed=3)
+    for t in range(4):
+        m = _closed_model(nicr, shape, 10 + t)
+        m.step(40)
+        for k in range(3):
+            rep = net.record(m, 20)
+            assert rep.status == 0
+    assert net.samples == 12
+    net.fit_norm()
+    mean, scale, out_scale = net.norm()
+    assert out_scale[2] == 0.0     # phi never changes in this workload
+    losses = net.train(300, batch=2, lr=1e-3, seed=1)
+    first, last = float(np.mean(losses[:10])), float(np.mean(losses[-30:]))
+    print(f"training loss {first:.4f} -> {last:.4f}")
+    assert np.all(np.isfinite(losses))
+    assert last < 0.5 * first
+    # surrogate stepping from a fresh state of the same generator
+    a = _closed_model(nicr, shape, 99)
+    b = _closed_model(nicr, shape, 99)
+    a.step(120)
+    b.step(120)
+    m0 = [sum(a.stats("o")[0]), sum(a.stats("cr")[0])]
+    rep = net.advance(a, 200, jump=20, residual_tol=0.3)
+    b.step(200)
+    print(f"jumps accepted {rep.jumps_accepted} rejected {rep.jumps_rejected} plain {rep.plain_steps}")
+    assert rep.status == 0 and rep.steps_done == 200
+    assert rep.plain_steps + 20 * rep.jumps_accepted == 200
+    m1 = [sum(a.stats("o")[0]), sum(a.stats("cr")[0])]
+    for x, y in zip(m0, m1):
+        assert abs(x - y) <= 1e-12 * abs(x)
+    for f in ("o", "cr"):
+        fa, fb = a.get(f), b.get(f)
+        rel = np.linalg.norm(fa - fb) / np.linalg.norm(fb)
+        print(f"{f}: rel L2 vs plain {rel:.3e}")
+        assert rel < 1e-2
+    net.close()
