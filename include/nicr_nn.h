@@ -11,8 +11,9 @@
  * Conventions as nicr.h: plain C, host arrays borrowed for one call, every
  * function returns NICR_OK or a negative NICR_E_* code (nicr_last_error()).
  * Host volumes are float32 in the model's cell order ((x*ny + y)*nz + z), one
- * channel after another: inputs [nb][4][V] (O, Cr, phi, gb), outputs and
- * targets [nb][3][V] (normalised changes of O, Cr, phi).
+ * channel after another: inputs [nb][7][V] (O, Cr, phi, gb and the plain
+ * one-step update of O, Cr, phi), outputs and targets [nb][3][V] (normalised
+ * corrections, see nicr_unet_norm).
  */
 #ifndef NICR_NN_H
 #define NICR_NN_H
@@ -41,11 +42,24 @@ int nicr_unet_get_params(void* net, float* host);
 int nicr_unet_set_params(void* net, const float* host);
 /* Gradients of the last nicr_unet_loss_backward (same layout as the parameters). */
 int nicr_unet_get_grads(void* net, float* host);
-/* Input normalisation x_c = (f_c - in_mean_c) / in_scale_c and output scale:
- * the change of field c is out_c * out_scale_c (out_scale_c = 0 freezes c). */
-int nicr_unet_set_norm(void* net, const float in_mean[4], const float in_scale[4], const float out_scale[3]);
-int nicr_unet_get_norm(void* net, float in_mean[4], float in_scale[4], float out_scale[3]);
-/* Forward pass of nb volumes: in [nb][4][V] -> out [nb][3][V] (normalised). */
+/* Normalisation. Each volume is seen relative to its own statistics (mean m
+ * and population standard deviation s of O and Cr, RMS r of the plain one-step
+ * update u of O, Cr, phi; binary64). Input channels: 0, 1 = (O - m_O) / s_O,
+ * (Cr - m_Cr) / s_Cr; 2, 3 = (phi - phi_mean) / phi_scale, (gb - gb_mean) /
+ * gb_scale; 4..6 = u_O / r_O, u_Cr / r_Cr, u_phi / r_phi (0 where the
+ * divisor is 0); 7..10 constant fields (c_k - c_mean[k]) / c_scale[k] of c =
+ * (log10 s_O, m_Cr, log10 r_O, log10 r_Cr); channels 11..31 are zero. The
+ * network predicts the correction to the linear extrapolation of the update
+ * over `jump` steps: change of field k = jump u_k + out_k jump r_k gain[k]. */
+typedef struct nicr_unet_norm {
+    float phi_mean, phi_scale, gb_mean, gb_scale;
+    float c_mean[4], c_scale[4];
+    float gain[3];
+    int32_t jump;
+} nicr_unet_norm;
+int nicr_unet_set_norm(void* net, const nicr_unet_norm* norm);
+int nicr_unet_get_norm(void* net, nicr_unet_norm* norm);
+/* Forward pass of nb volumes: in [nb][7][V] -> out [nb][3][V] (normalised). */
 int nicr_unet_forward(void* net, int nb, const float* in, float* out);
 /* Forward pass, then the gradients of L = (1/3) sum_p sum_c (out - target)^2
  * with respect to every parameter (overwritten, not accumulated); *loss = L
@@ -64,14 +78,16 @@ int nicr_nn_conv_check(int op, int nb, int s0, int s1, int s2, int cin, int cout
                        const float* w, const float* b, const float* dy, float* out);
 
 /* ---- training set and training ---- */
-/* Record one training sample from a binary64 3D model of the network's volume:
- * its current (O, Cr, phi, mask) and the change after `jump` steps, which this
- * call takes on the model (nicr_step). A rejected step records nothing and is
- * reported in *report. */
+/* Record one training sample from a binary64 3D model of the network's volume.
+ * The call takes 1 + jump steps on the model (nicr_step): the first gives the
+ * plain update u; the sample holds the state after it (O, Cr, phi, mask), u,
+ * and the change over the next `jump` steps. All samples of a set share
+ * `jump`. A rejected step records nothing and is reported in *report. */
 int nicr_sur_record(void* net, void* model, int jump, nicr_step_report* report);
 int nicr_sur_samples(void* net, int* count);
-/* Normalisation from the training set (input mean and standard deviation,
- * output RMS). */
+/* Normalisation from the training set: phi and gb means and standard
+ * deviations, means and standard deviations of the volume statistics, gains
+ * that give the normalised corrections unit mean square, and the jump. */
 int nicr_sur_fit_norm(void* net);
 /* `iters` Adam iterations (learning rate lr) on random batches of `batch`
  * samples drawn with `seed`; losses[iters] (mean squared error of the
@@ -101,11 +117,12 @@ typedef struct nicr_sur_report {
 
 /* Advance a binary64 3D model by nsteps with network jumps of `jump` steps.
  * Each cycle: one plain step x_-1 -> x0 (update u0 = x0 - x_-1); the proposal
- * x0 + network(x0) for x0 after `jump` steps (eta fields are held); optionally
- * O and Cr rescaled to their totals at x0; one plain check step from the
- * proposal xh to xh1. The jump is accepted when O, Cr, phi are inside
- * [min, max], the model accepts the check step, and for O and Cr the trapezoid
- * residual R = (xh - x0) - (jump / 2) (u0 + (xh1 - xh)) satisfies
+ * xh = x0 + jump u0 + correction(x0, u0) for the state `jump` steps later (eta
+ * fields are held; jump must equal the network's); optionally O and Cr
+ * rescaled to their totals at x0; one plain check step from xh to xh1. The
+ * jump is accepted when O, Cr, phi are inside [min, max], the model accepts the
+ * check step, and for O and Cr the trapezoid residual
+ * R = (xh - x0) - (jump / 2) (u0 + (xh1 - xh)) satisfies
  * ||R|| <= residual_tol ||xh - x0|| (a proposal that does not move fails).
  * Otherwise the model is put back exactly to x0 (fields, state index, budgets)
  * and takes the `jump` plain steps instead. */

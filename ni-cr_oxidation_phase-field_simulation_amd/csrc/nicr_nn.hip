@@ -22,6 +22,7 @@
 #include "nicr_nn_kernels.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <random>
 
 namespace nicr {
@@ -29,7 +30,10 @@ namespace nn {
 
 constexpr int N_CONV = 14;
 constexpr int HEAD_IN = 32;
-constexpr int N_IN = 4, N_OUT = 3;
+constexpr int N_FIELD = 4;            // O, Cr, phi, mask
+constexpr int N_OUT = 3;              // changes of O, Cr, phi
+constexpr int N_IN = N_FIELD + N_OUT; // stored per volume: the fields and the plain update of O, Cr, phi
+constexpr int N_STAT = 7;             // mean O, std O, mean Cr, std Cr, rms update O, Cr, phi
 
 // ---- head, loss, inputs, optimiser ----
 
@@ -116,47 +120,119 @@ __global__ __launch_bounds__(HB_THREADS) void k_head_bwd(int P, const bf16* __re
     }
 }
 
-struct Norm {
-    float mean[N_IN], inv[N_IN];   // x0 channel c = (field_c - mean_c) * inv_c
-    float out_scale[N_OUT];        // delta_c = out_c * out_scale_c
-    float out_inv[N_OUT];          // target_c = delta_c * out_inv_c (0 where out_scale is 0)
+// Input and output normalisation. The network sees each volume relative to its
+// own statistics. Input channels: 0, 1 the deviations of O and Cr from their
+// volume means over their volume standard deviations (formed in binary64
+// before rounding to BF16, so the small deviations that drive the change keep
+// their precision); 2, 3 phi and the mask under fixed scales; 4..6 the plain
+// one-step update u of O, Cr, phi at the volume (the step just before the
+// jump) over its volume RMS; 7..10 constant fields carrying log10 std O, mean
+// Cr, log10 rms u_O, log10 rms u_Cr under fixed scales. The network predicts
+// the correction to the linear extrapolation J u of the update over the J
+// steps of a jump: change_c = J u_c + out_c J rms(u_c) gain_c (a field whose
+// update is zero does not change).
+constexpr int N_CST = 4;
+struct VolNorm {
+    double mean[N_FIELD];          // channel c = (field_c - mean_c) * inv_c, c < 4
+    double inv[N_FIELD];
+    double uinv[N_OUT];            // channel 4 + c = u_c * uinv_c
+    float cst[N_CST];              // channels 7..10
+    double jump;                   // J
+    double out_scale[N_OUT];       // change_c = J u_c + out_c * out_scale_c
+    double out_inv[N_OUT];         // target_c = (change_c - J u_c) * out_inv_c (0 where out_scale is 0)
 };
 
-// Network input of one volume: x0[p][c] = (f_c[p] - mean_c) * inv_c for c < 4, 0 above.
-template <class T0, class T3>
+// Network input of one volume.
+template <class T0, class T3, class TU>
 __global__ void k_input(int V, const T0* __restrict__ f0, const T0* __restrict__ f1, const T0* __restrict__ f2,
-                        const T3* __restrict__ f3, Norm nm, bf16* __restrict__ x0) {
+                        const T3* __restrict__ f3, const TU* __restrict__ u0, const TU* __restrict__ u1,
+                        const TU* __restrict__ u2, VolNorm vn, bf16* __restrict__ x0) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= V) return;
-    bf16x8 v;
-    v[0] = (bf16)(((float)f0[p] - nm.mean[0]) * nm.inv[0]);
-    v[1] = (bf16)(((float)f1[p] - nm.mean[1]) * nm.inv[1]);
-    v[2] = (bf16)(((float)f2[p] - nm.mean[2]) * nm.inv[2]);
-    v[3] = (bf16)(((float)f3[p] - nm.mean[3]) * nm.inv[3]);
+    bf16x8 v, w;
+    v[0] = (bf16)(float)(((double)f0[p] - vn.mean[0]) * vn.inv[0]);
+    v[1] = (bf16)(float)(((double)f1[p] - vn.mean[1]) * vn.inv[1]);
+    v[2] = (bf16)(float)(((double)f2[p] - vn.mean[2]) * vn.inv[2]);
+    v[3] = (bf16)(float)(((double)f3[p] - vn.mean[3]) * vn.inv[3]);
+    v[4] = (bf16)(float)((double)u0[p] * vn.uinv[0]);
+    v[5] = (bf16)(float)((double)u1[p] * vn.uinv[1]);
+    v[6] = (bf16)(float)((double)u2[p] * vn.uinv[2]);
+    v[7] = (bf16)vn.cst[0];
+    w[0] = (bf16)vn.cst[1];
+    w[1] = (bf16)vn.cst[2];
+    w[2] = (bf16)vn.cst[3];
 #pragma unroll
-    for (int i = 4; i < 8; ++i) v[i] = (bf16)0.f;
+    for (int i = 3; i < 8; ++i) w[i] = (bf16)0.f;
     uint4* dst = reinterpret_cast<uint4*>(x0 + 32LL * p);
     dst[0] = __builtin_bit_cast(uint4, v);
-    dst[1] = dst[2] = dst[3] = make_uint4(0u, 0u, 0u, 0u);
+    dst[1] = __builtin_bit_cast(uint4, w);
+    dst[2] = dst[3] = make_uint4(0u, 0u, 0u, 0u);
 }
 
-// Training target of one volume: tgt[p] = (d0, d1, d2) * out_inv, row pitch 4.
-__global__ void k_target(int V, const float* __restrict__ d, Norm nm, float* __restrict__ tgt) {
+// Training target of one volume: tgt[p][c] = (d_c - J u_c) * out_inv_c, row pitch 4.
+__global__ void k_target(int V, const float* __restrict__ d, const float* __restrict__ u, VolNorm vn,
+                         float* __restrict__ tgt) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= V) return;
-    *reinterpret_cast<float4*>(tgt + 4LL * p) =
-        make_float4(d[p] * nm.out_inv[0], d[V + p] * nm.out_inv[1], d[2LL * V + p] * nm.out_inv[2], 0.f);
+    float t[N_OUT];
+#pragma unroll
+    for (int c = 0; c < N_OUT; ++c)
+        t[c] = (float)(((double)d[(long long)c * V + p] - vn.jump * (double)u[(long long)c * V + p]) * vn.out_inv[c]);
+    *reinterpret_cast<float4*>(tgt + 4LL * p) = make_float4(t[0], t[1], t[2], 0.f);
 }
 
-// Surrogate jump: field_c += out[p][c] * out_scale_c (binary64 state).
-__global__ void k_apply_jump(int V, const float* __restrict__ out, Norm nm, double* __restrict__ o,
+// Surrogate jump: field_c += J u_c + out[p][c] * out_scale_c (binary64 state).
+__global__ void k_apply_jump(int V, const float* __restrict__ out, VolNorm vn, const double* __restrict__ uo,
+                             const double* __restrict__ uc, const double* __restrict__ up, double* __restrict__ o,
                              double* __restrict__ cr, double* __restrict__ ph) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= V) return;
     const float4 v = *reinterpret_cast<const float4*>(out + 4LL * p);
-    o[p] += (double)v.x * (double)nm.out_scale[0];
-    cr[p] += (double)v.y * (double)nm.out_scale[1];
-    ph[p] += (double)v.z * (double)nm.out_scale[2];
+    o[p] += vn.jump * uo[p] + (double)v.x * vn.out_scale[0];
+    cr[p] += vn.jump * uc[p] + (double)v.y * vn.out_scale[1];
+    ph[p] += vn.jump * up[p] + (double)v.z * vn.out_scale[2];
+}
+
+// acc += sum (d - J u)^2 (float arrays; binary64 sums)
+__global__ __launch_bounds__(256) void k_corr_sq(long long n, const float* __restrict__ d, const float* __restrict__ u,
+                                                 double J, double* __restrict__ acc) {
+    __shared__ double red[4];
+    double s = 0.;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double r = (double)d[i] - J * (double)u[i];
+        s += r * r;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(acc, red[0] + red[1] + red[2] + red[3]);
+}
+
+// acc[0] += sum a, acc[1] += sum a^2 (binary64), for volume statistics.
+__global__ __launch_bounds__(256) void k_moments_d(long long n, const double* __restrict__ a,
+                                                   double* __restrict__ acc) {
+    __shared__ double red[2][4];
+    double s = 0., t = 0.;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double v = a[i];
+        s += v;
+        t += v * v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        t += __shfl_xor(t, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = s;
+        red[1][threadIdx.x >> 6] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&acc[0], red[0][0] + red[0][1] + red[0][2] + red[0][3]);
+        atomicAdd(&acc[1], red[1][0] + red[1][1] + red[1][2] + red[1][3]);
+    }
 }
 
 __global__ void k_scale(long long n, double* __restrict__ a, double f) {
@@ -322,13 +398,15 @@ struct Net {
     float* loss_acc = nullptr;   // one slot per training iteration
     int loss_cap = 0;
     double* dacc = nullptr;      // reduction scratch
-    Norm nm{};
+    nicr_unet_norm norm{};
+    std::vector<double> ds_stat;   // per training sample: mean and std of O and Cr
     // training set: inputs [S][4][V] and 50-step deltas [S][3][V] (FP32)
     int ds_cap = 0, ds_n = 0;
     float *ds_in = nullptr, *ds_d = nullptr;
     // binary64 state backup of a surrogate jump (O, Cr, phi)
     double* bk[3] = {nullptr, nullptr, nullptr};
-    double* du[2] = {nullptr, nullptr};   // plain update of O and Cr before a jump
+    double* du[3] = {nullptr, nullptr, nullptr};   // plain update of O, Cr, phi before a jump
+    int ds_jump = 0;                                 // steps of the recorded changes
     long long bk_cells = 0;
     bool timing = false;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -374,7 +452,49 @@ Geo geo(const Net* n, int nb, int level, int cin, int cout, int taps) {
     return g;
 }
 
+// Halo-tile kernel settings: compute units of the current device (one
+// resident block each: 124 KB of LDS) and whether the kernel may be used
+// (NICR_CONV_HALO=0 keeps the per-tap kernel everywhere).
+struct HaloCfg {
+    int ncu = 0;
+    bool enabled = true;
+    bool attr_set = false;
+};
+static HaloCfg g_halo;
+
+static bool halo_ready() {
+    if (!g_halo.attr_set) {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+            return false;
+        g_halo.ncu = ncu;
+        const char* e = getenv("NICR_CONV_HALO");
+        g_halo.enabled = !(e && e[0] == '0');
+        if (hipFuncSetAttribute((const void*)k_conv_halo<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                HC_LDS_BYTES) != hipSuccess ||
+            hipFuncSetAttribute((const void*)k_conv_halo<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                HC_LDS_BYTES) != hipSuccess)
+            g_halo.enabled = false;
+        g_halo.attr_set = true;
+    }
+    return g_halo.enabled;
+}
+
 void launch_conv(Net* n, const Geo& g, const bf16* X, const bf16* W, const float* bias, bf16* Y, bool relu) {
+    if (g.s0 % HT_X == 0 && g.s1 % HT_Y == 0 && g.s2 % HT_Z == 0 && halo_ready()) {
+        const int units = g.nb * (g.s0 / HT_X) * (g.s1 / HT_Y) * (g.s2 / HT_Z) * (g.cout / 32);
+        const int nblock = std::min(units, g_halo.ncu);
+        const int upb = (units + nblock - 1) / nblock;
+        const int grid = (units + upb - 1) / upb;
+        if (relu)
+            hipLaunchKernelGGL(k_conv_halo<true>, dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W, bias,
+                               Y, units, upb);
+        else
+            hipLaunchKernelGGL(k_conv_halo<false>, dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W,
+                               bias, Y, units, upb);
+        return;
+    }
     const int P = g.positions();
     const unsigned mb = (unsigned)((P + CBM - 1) / CBM);
     if (g.cout % 64 == 0) {
@@ -483,11 +603,10 @@ int net_build(Net* n) {
     wire(c[11], n->d2a, n->d2, n->g_d2a, n->g_d2);
     wire(c[12], n->u1, n->d1a, n->g_u1, n->g_d1a);
     wire(c[13], n->d1a, n->d1, n->g_d1a, n->g_d1);
-    for (int i = 0; i < N_IN; ++i) {
-        n->nm.mean[i] = 0.f;
-        n->nm.inv[i] = 1.f;
-    }
-    for (int i = 0; i < N_OUT; ++i) n->nm.out_scale[i] = n->nm.out_inv[i] = 1.f;
+    n->norm = nicr_unet_norm{};
+    n->norm.phi_scale = n->norm.gb_scale = 1.f;
+    for (int i = 0; i < N_CST; ++i) n->norm.c_scale[i] = 1.f;
+    for (int i = 0; i < N_OUT; ++i) n->norm.gain[i] = 1.f;
     return NICR_OK;
 }
 
@@ -632,15 +751,66 @@ void adam_step(Net* n, float lr) {
     cast_weights(n);
 }
 
+// volume statistics (mean O, std O, mean Cr, std Cr, rms u_O, rms u_Cr, rms
+// u_phi) -> the kernels' normalisation
+VolNorm vol_norm(const nicr_unet_norm& nm, const double st[N_STAT]) {
+    VolNorm v{};
+    const double mo = st[0], so = st[1], mc = st[2], sc = st[3];
+    v.mean[0] = mo;
+    v.inv[0] = so > 0. ? 1. / so : 0.;
+    v.mean[1] = mc;
+    v.inv[1] = sc > 0. ? 1. / sc : 0.;
+    v.mean[2] = nm.phi_mean;
+    v.inv[2] = 1. / nm.phi_scale;
+    v.mean[3] = nm.gb_mean;
+    v.inv[3] = 1. / nm.gb_scale;
+    for (int k = 0; k < N_OUT; ++k) v.uinv[k] = st[4 + k] > 0. ? 1. / st[4 + k] : 0.;
+    const double c[N_CST] = {std::log10(std::max(so, 1e-300)), mc, std::log10(std::max(st[4], 1e-300)),
+                             std::log10(std::max(st[5], 1e-300))};
+    for (int k = 0; k < N_CST; ++k) v.cst[k] = (float)((c[k] - nm.c_mean[k]) / nm.c_scale[k]);
+    v.jump = (double)nm.jump;
+    for (int k = 0; k < N_OUT; ++k) {
+        v.out_scale[k] = v.jump * st[4 + k] * (double)nm.gain[k];
+        v.out_inv[k] = v.out_scale[k] > 0. ? 1. / v.out_scale[k] : 0.;
+    }
+    return v;
+}
+
+// sums s[0..3] = sum O, sum O^2, sum Cr, sum Cr^2 and s[4..6] = sum u_c^2 -> statistics
+void stat_from_sums(const double s[N_STAT], double V, double st[N_STAT]) {
+    for (int f = 0; f < 2; ++f) {
+        const double m = s[2 * f] / V;
+        st[2 * f] = m;
+        st[2 * f + 1] = std::sqrt(std::max(0., s[2 * f + 1] / V - m * m));
+    }
+    for (int k = 0; k < N_OUT; ++k) st[4 + k] = std::sqrt(s[4 + k] / V);
+}
+
+// statistics of a binary64 volume and its update on the device
+int device_stats(Net* n, const double* o, const double* cr, double* const u[N_OUT], long long V, double st[N_STAT]) {
+    HIPCHK(hipMemsetAsync(n->dacc, 0, 10 * 8, n->stream));
+    hipLaunchKernelGGL(k_moments_d, dim3(512), dim3(256), 0, n->stream, V, o, n->dacc);
+    hipLaunchKernelGGL(k_moments_d, dim3(512), dim3(256), 0, n->stream, V, cr, n->dacc + 2);
+    for (int k = 0; k < N_OUT; ++k)
+        hipLaunchKernelGGL(k_moments_d, dim3(512), dim3(256), 0, n->stream, V, (const double*)u[k], n->dacc + 4 + 2 * k);
+    double a[10];
+    HIPCHK(hipMemcpyAsync(a, n->dacc, 10 * 8, hipMemcpyDeviceToHost, n->stream));
+    HIPCHK(hipStreamSynchronize(n->stream));
+    const double s[N_STAT] = {a[0], a[1], a[2], a[3], a[5], a[7], a[9]};
+    stat_from_sums(s, (double)V, st);
+    return NICR_OK;
+}
+
 // network input and target of batch slot `slot` from training sample `si`
 void load_sample(Net* n, int slot, int si, bool target) {
     const long long V = n->V(0);
     const float* in = n->ds_in + (long long)si * N_IN * V;
-    hipLaunchKernelGGL((k_input<float, float>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, in, in + V, in + 2 * V,
-                       in + 3 * V, n->nm, n->x0 + (long long)slot * V * 32);
+    const VolNorm vn = vol_norm(n->norm, &n->ds_stat[N_STAT * (size_t)si]);
+    hipLaunchKernelGGL((k_input<float, float, float>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, in, in + V,
+                       in + 2 * V, in + 3 * V, in + 4 * V, in + 5 * V, in + 6 * V, vn, n->x0 + (long long)slot * V * 32);
     if (target)
         hipLaunchKernelGGL(k_target, dim3(nblk(V)), dim3(256), 0, n->stream, (int)V,
-                           n->ds_d + (long long)si * N_OUT * V, n->nm, n->tgt + 4LL * slot * V);
+                           n->ds_d + (long long)si * N_OUT * V, in + 4 * V, vn, n->tgt + 4LL * slot * V);
 }
 
 int check_ctx_shape(Net* n, Ctx* c) {
@@ -650,8 +820,10 @@ int check_ctx_shape(Net* n, Ctx* c) {
     return NICR_OK;
 }
 
-// network input from the model's current state (binary64 O, Cr, phi and the mask)
-int input_from_ctx(Net* n, Ctx* c, int slot) {
+// network input from the model's current state (binary64 O, Cr, phi, the
+// mask, and the plain update held in n->du); *vn receives the volume's
+// normalisation
+int input_from_ctx(Net* n, Ctx* c, int slot, VolNorm* vn) {
     if (c->mask_dirty) {
         do_mask(c);
         c->mask_dirty = false;
@@ -659,13 +831,19 @@ int input_from_ctx(Net* n, Ctx* c, int slot) {
     HIPCHK(hipStreamSynchronize(c->stream));
     const long long V = n->V(0);
     const int cur = c->cur;
+    double st[N_STAT];
+    int rc = device_stats(n, c->o[cur], c->cr[cur], n->du, V, st);
+    if (rc) return rc;
+    *vn = vol_norm(n->norm, st);
     bf16* x0 = n->x0 + (long long)slot * V * 32;
     if (c->te == 8)
-        hipLaunchKernelGGL((k_input<double, double>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, c->o[cur],
-                           c->cr[cur], c->p[cur], (const double*)c->gb[mask_index(c)], n->nm, x0);
+        hipLaunchKernelGGL((k_input<double, double, double>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V,
+                           c->o[cur], c->cr[cur], c->p[cur], (const double*)c->gb[mask_index(c)], n->du[0], n->du[1],
+                           n->du[2], *vn, x0);
     else
-        hipLaunchKernelGGL((k_input<double, float>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, c->o[cur],
-                           c->cr[cur], c->p[cur], (const float*)c->gb[mask_index(c)], n->nm, x0);
+        hipLaunchKernelGGL((k_input<double, float, double>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V,
+                           c->o[cur], c->cr[cur], c->p[cur], (const float*)c->gb[mask_index(c)], n->du[0], n->du[1],
+                           n->du[2], *vn, x0);
     return NICR_OK;
 }
 
@@ -761,30 +939,22 @@ int nicr_unet_get_grads(void* h, float* host) {
     return NICR_OK;
 }
 
-int nicr_unet_set_norm(void* h, const float in_mean[4], const float in_scale[4], const float out_scale[3]) {
+int nicr_unet_set_norm(void* h, const nicr_unet_norm* nm) {
     Net* n = static_cast<Net*>(h);
-    if (!n || !in_mean || !in_scale || !out_scale) return fail(NICR_E_INVALID, "null argument");
-    for (int i = 0; i < N_IN; ++i) {
-        if (!(in_scale[i] > 0.f)) return fail(NICR_E_INVALID, "nicr_unet_set_norm: input scales must be > 0");
-        n->nm.mean[i] = in_mean[i];
-        n->nm.inv[i] = 1.f / in_scale[i];
-    }
-    for (int i = 0; i < N_OUT; ++i) {
-        if (!(out_scale[i] >= 0.f)) return fail(NICR_E_INVALID, "nicr_unet_set_norm: output scales must be >= 0");
-        n->nm.out_scale[i] = out_scale[i];
-        n->nm.out_inv[i] = out_scale[i] > 0.f ? 1.f / out_scale[i] : 0.f;
-    }
+    if (!n || !nm) return fail(NICR_E_INVALID, "null argument");
+    if (!(nm->phi_scale > 0.f && nm->gb_scale > 0.f)) return fail(NICR_E_INVALID, "nicr_unet_set_norm: scales must be > 0");
+    for (int k = 0; k < N_CST; ++k)
+        if (!(nm->c_scale[k] > 0.f)) return fail(NICR_E_INVALID, "nicr_unet_set_norm: scales must be > 0");
+    for (int k = 0; k < N_OUT; ++k)
+        if (!(nm->gain[k] >= 0.f)) return fail(NICR_E_INVALID, "nicr_unet_set_norm: gains must be >= 0");
+    n->norm = *nm;
     return NICR_OK;
 }
 
-int nicr_unet_get_norm(void* h, float in_mean[4], float in_scale[4], float out_scale[3]) {
+int nicr_unet_get_norm(void* h, nicr_unet_norm* nm) {
     Net* n = static_cast<Net*>(h);
-    if (!n || !in_mean || !in_scale || !out_scale) return fail(NICR_E_INVALID, "null argument");
-    for (int i = 0; i < N_IN; ++i) {
-        in_mean[i] = n->nm.mean[i];
-        in_scale[i] = 1.f / n->nm.inv[i];
-    }
-    for (int i = 0; i < N_OUT; ++i) out_scale[i] = n->nm.out_scale[i];
+    if (!n || !nm) return fail(NICR_E_INVALID, "null argument");
+    *nm = n->norm;
     return NICR_OK;
 }
 
@@ -797,11 +967,25 @@ int nicr_unet_forward(void* h, int nb, const float* in, float* out) {
     HIPCHK(hipMalloc(&tmp, (size_t)V * N_IN * 4));
     int rc = NICR_OK;
     for (int b = 0; b < nb && !rc; ++b) {
-        if (hipMemcpyAsync(tmp, in + (size_t)b * N_IN * V, (size_t)V * N_IN * 4, hipMemcpyHostToDevice, n->stream) !=
-            hipSuccess)
+        const float* src = in + (size_t)b * N_IN * V;
+        double s[N_STAT] = {0., 0., 0., 0., 0., 0., 0.}, st[N_STAT];
+        for (int f = 0; f < 2; ++f)
+            for (long long p = 0; p < V; ++p) {
+                const double v = src[(size_t)f * V + p];
+                s[2 * f] += v;
+                s[2 * f + 1] += v * v;
+            }
+        for (int k = 0; k < N_OUT; ++k)
+            for (long long p = 0; p < V; ++p) {
+                const double v = src[(size_t)(N_FIELD + k) * V + p];
+                s[4 + k] += v * v;
+            }
+        stat_from_sums(s, (double)V, st);
+        if (hipMemcpyAsync(tmp, src, (size_t)V * N_IN * 4, hipMemcpyHostToDevice, n->stream) != hipSuccess)
             rc = fail(NICR_E_DEVICE, "nicr_unet_forward: upload failed");
-        hipLaunchKernelGGL((k_input<float, float>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, tmp, tmp + V,
-                           tmp + 2 * V, tmp + 3 * V, n->nm, n->x0 + (long long)b * V * 32);
+        hipLaunchKernelGGL((k_input<float, float, float>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, tmp,
+                           tmp + V, tmp + 2 * V, tmp + 3 * V, tmp + 4 * V, tmp + 5 * V, tmp + 6 * V,
+                           vol_norm(n->norm, st), n->x0 + (long long)b * V * 32);
         if (hipStreamSynchronize(n->stream) != hipSuccess) rc = fail(NICR_E_DEVICE, "nicr_unet_forward: input");
     }
     if (!rc) {
@@ -949,14 +1133,22 @@ int nicr_sur_record(void* h, void* model, int jump, nicr_step_report* rep) {
     int rc = check_ctx_shape(n, c);
     if (rc) return rc;
     if (n->ds_n >= n->ds_cap) return fail(NICR_E_STATE, "nicr_sur_record: training set is full");
-    if (c->mask_dirty) {
-        do_mask(c);
-        c->mask_dirty = false;
-    }
+    if (n->ds_n > 0 && jump != n->ds_jump)
+        return fail(NICR_E_INVALID, "nicr_sur_record: every sample of a training set has the same jump");
     const long long V = n->V(0);
     if ((rc = sur_buffers(n))) return rc;
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (rep) *rep = nicr_step_report{};
+    // the plain step whose update enters the input
+    nicr_step_report r{};
+    if ((rc = nicr_step(c, 1, &r))) return rc;
+    if (r.status != NICR_STEP_OK) {
+        if (rep) *rep = r;
+        return NICR_OK;
+    }
     const int cur = c->cur;
+    hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], c->o[cur ^ 1], n->du[0]);
+    hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[cur], c->cr[cur ^ 1], n->du[1]);
+    hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->p[cur], c->p[cur ^ 1], n->du[2]);
     float* in = n->ds_in + (long long)n->ds_n * N_IN * V;
     hipLaunchKernelGGL(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], in);
     hipLaunchKernelGGL(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[cur], in + V);
@@ -966,11 +1158,14 @@ int nicr_sur_record(void* h, void* model, int jump, nicr_step_report* rep) {
                            (const double*)c->gb[mask_index(c)], in + 3 * V);
     else
         HIPCHK(hipMemcpyAsync(in + 3 * V, c->gb[mask_index(c)], V * 4, hipMemcpyDeviceToDevice, n->stream));
+    for (int k = 0; k < N_OUT; ++k)
+        hipLaunchKernelGGL(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, (const double*)n->du[k],
+                           in + (N_FIELD + k) * V);
     HIPCHK(hipMemcpyAsync(n->bk[0], c->o[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
     HIPCHK(hipMemcpyAsync(n->bk[1], c->cr[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
     HIPCHK(hipMemcpyAsync(n->bk[2], c->p[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
-    HIPCHK(hipStreamSynchronize(n->stream));
-    nicr_step_report r{};
+    double st[N_STAT];
+    if ((rc = device_stats(n, c->o[cur], c->cr[cur], n->du, V, st))) return rc;
     if ((rc = nicr_step(c, jump, &r))) return rc;
     if (rep) *rep = r;
     if (r.status != NICR_STEP_OK) return NICR_OK;   // not recorded
@@ -980,6 +1175,9 @@ int nicr_sur_record(void* h, void* model, int jump, nicr_step_report* rep) {
     hipLaunchKernelGGL(k_delta_float, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[nc], n->bk[1], d + V);
     hipLaunchKernelGGL(k_delta_float, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->p[nc], n->bk[2], d + 2 * V);
     HIPCHK(hipStreamSynchronize(n->stream));
+    n->ds_stat.resize(N_STAT * (size_t)(n->ds_n + 1));
+    for (int k = 0; k < N_STAT; ++k) n->ds_stat[N_STAT * (size_t)n->ds_n + k] = st[k];
+    n->ds_jump = jump;
     ++n->ds_n;
     return NICR_OK;
 }
@@ -991,36 +1189,70 @@ int nicr_sur_samples(void* h, int* count) {
     return NICR_OK;
 }
 
-// Normalisation from the training set: input channel mean and standard
-// deviation, output channel RMS of the deltas (0 for a channel that never changes).
+// Normalisation from the training set: phi and mask means and standard
+// deviations over all samples, means and standard deviations of the volume
+// statistics over the samples, and output gains that give each normalised
+// correction unit mean square (0 for a field that never changes).
 int nicr_sur_fit_norm(void* h) {
     Net* n = static_cast<Net*>(h);
     if (!n || n->ds_n < 1) return fail(NICR_E_STATE, "nicr_sur_fit_norm: empty training set");
     const long long V = n->V(0);
-    HIPCHK(hipMemsetAsync(n->dacc, 0, 64 * 8, n->stream));
-    for (int s = 0; s < n->ds_n; ++s) {
-        for (int c = 0; c < N_IN; ++c)
-            hipLaunchKernelGGL(k_moments, dim3(256), dim3(256), 0, n->stream, V,
-                               n->ds_in + ((long long)s * N_IN + c) * V, n->dacc + 2 * c);
+    const int S = n->ds_n;
+    const double J = (double)n->ds_jump;
+    // per sample: moments of phi and gb, squared corrections of the three fields
+    constexpr int NM = 7;
+    std::vector<double> m((size_t)S * NM);
+    for (int s = 0; s < S; ++s) {
+        HIPCHK(hipMemsetAsync(n->dacc, 0, NM * 8, n->stream));
+        const float* in = n->ds_in + (long long)s * N_IN * V;
+        for (int c = 0; c < 2; ++c)
+            hipLaunchKernelGGL(k_moments, dim3(256), dim3(256), 0, n->stream, V, in + (2 + c) * V, n->dacc + 2 * c);
         for (int c = 0; c < N_OUT; ++c)
-            hipLaunchKernelGGL(k_moments, dim3(256), dim3(256), 0, n->stream, V,
-                               n->ds_d + ((long long)s * N_OUT + c) * V, n->dacc + 2 * (N_IN + c));
+            hipLaunchKernelGGL(k_corr_sq, dim3(256), dim3(256), 0, n->stream, V,
+                               n->ds_d + ((long long)s * N_OUT + c) * V, in + (N_FIELD + c) * V, J, n->dacc + 4 + c);
+        HIPCHK(hipMemcpyAsync(&m[(size_t)s * NM], n->dacc, NM * 8, hipMemcpyDeviceToHost, n->stream));
     }
-    double m[2 * (N_IN + N_OUT)];
-    HIPCHK(hipMemcpyAsync(m, n->dacc, sizeof(m), hipMemcpyDeviceToHost, n->stream));
     HIPCHK(hipStreamSynchronize(n->stream));
-    const double cnt = (double)V * n->ds_n;
-    float mean[N_IN], sc[N_IN], os[N_OUT];
-    for (int c = 0; c < N_IN; ++c) {
-        const double mu = m[2 * c] / cnt, var = std::max(0., m[2 * c + 1] / cnt - mu * mu);
-        mean[c] = (float)mu;
-        sc[c] = var > 1e-30 ? (float)std::sqrt(var) : 1.f;
+    nicr_unet_norm nm{};
+    auto mean_std = [](double s1, double s2, double cnt, float* mean, float* sd) {
+        const double mu = s1 / cnt, var = std::max(0., s2 / cnt - mu * mu);
+        *mean = (float)mu;
+        *sd = var > 1e-30 ? (float)std::sqrt(var) : 1.f;
+    };
+    double a[4] = {0., 0., 0., 0.};
+    for (int s = 0; s < S; ++s)
+        for (int k = 0; k < 4; ++k) a[k] += m[(size_t)s * NM + k];
+    mean_std(a[0], a[1], (double)V * S, &nm.phi_mean, &nm.phi_scale);
+    mean_std(a[2], a[3], (double)V * S, &nm.gb_mean, &nm.gb_scale);
+    double cs[N_CST][2] = {};
+    for (int s = 0; s < S; ++s) {
+        const double* st = &n->ds_stat[N_STAT * (size_t)s];
+        const double c[N_CST] = {std::log10(std::max(st[1], 1e-300)), st[2], std::log10(std::max(st[4], 1e-300)),
+                                 std::log10(std::max(st[5], 1e-300))};
+        for (int k = 0; k < N_CST; ++k) {
+            cs[k][0] += c[k];
+            cs[k][1] += c[k] * c[k];
+        }
     }
-    for (int c = 0; c < N_OUT; ++c) {
-        const double rms = std::sqrt(m[2 * (N_IN + c) + 1] / cnt);
-        os[c] = rms > 1e-300 ? (float)rms : 0.f;
+    for (int k = 0; k < N_CST; ++k) mean_std(cs[k][0], cs[k][1], (double)S, &nm.c_mean[k], &nm.c_scale[k]);
+    double g[N_OUT] = {0., 0., 0.};
+    long long cnt[N_OUT] = {0, 0, 0};
+    for (int s = 0; s < S; ++s) {
+        const double* st = &n->ds_stat[N_STAT * (size_t)s];
+        for (int k = 0; k < N_OUT; ++k) {
+            const double scale = J * st[4 + k];
+            if (scale > 0.) {
+                g[k] += m[(size_t)s * NM + 4 + k] / (scale * scale);
+                cnt[k] += V;
+            }
+        }
     }
-    return nicr_unet_set_norm(h, mean, sc, os);
+    for (int k = 0; k < N_OUT; ++k) {
+        const double rms = cnt[k] ? std::sqrt(g[k] / (double)cnt[k]) : 0.;
+        nm.gain[k] = rms > 1e-300 ? (float)rms : 0.f;
+    }
+    nm.jump = n->ds_jump;
+    return nicr_unet_set_norm(h, &nm);
 }
 
 // `iters` Adam iterations on random batches of `batch` training samples. The
@@ -1098,7 +1330,7 @@ static int sur_buffers(Net* n) {
     int rc = 0;
     for (int i = 0; i < 3; ++i)
         if ((rc = nalloc_t(n, &n->bk[i], V))) return rc;
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < N_OUT; ++i)
         if ((rc = nalloc_t(n, &n->du[i], V))) return rc;
     n->bk_cells = V;
     return NICR_OK;
@@ -1124,6 +1356,7 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
     if (rc) return rc;
     const int J = opt->jump;
     if (J < 2) return fail(NICR_E_INVALID, "nicr_sur_advance: jump must be >= 2");
+    if (J != n->norm.jump) return fail(NICR_E_INVALID, "nicr_sur_advance: jump differs from the network's");
     if (!(opt->residual_tol > 0.)) return fail(NICR_E_INVALID, "nicr_sur_advance: residual_tol must be > 0");
     if ((rc = sur_buffers(n))) return rc;
     const long long V = n->V(0);
@@ -1155,6 +1388,7 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         HIPCHK(hipStreamSynchronize(c->stream));
         hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], c->o[cur ^ 1], n->du[0]);
         hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[cur], c->cr[cur ^ 1], n->du[1]);
+        hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->p[cur], c->p[cur ^ 1], n->du[2]);
         HIPCHK(hipMemcpyAsync(n->bk[0], c->o[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
         HIPCHK(hipMemcpyAsync(n->bk[1], c->cr[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
         HIPCHK(hipMemcpyAsync(n->bk[2], c->p[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
@@ -1164,9 +1398,11 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         DevCtl saved;
         if ((rc = sync_ctl(c, &saved))) return rc;
         // proposal x0 + network(x0)
-        if ((rc = input_from_ctx(n, c, 0))) return rc;
+        VolNorm vn;
+        if ((rc = input_from_ctx(n, c, 0, &vn))) return rc;
         forward(n, 1);
-        hipLaunchKernelGGL(k_apply_jump, dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, n->out, n->nm, c->o[cur],
+        hipLaunchKernelGGL(k_apply_jump, dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, n->out, vn, n->du[0],
+                           n->du[1], n->du[2], c->o[cur],
                            c->cr[cur], c->p[cur]);
         HIPCHK(hipStreamSynchronize(n->stream));
         if (n->timing) {

@@ -158,6 +158,156 @@ __global__ __launch_bounds__(256) void k_conv(Geo g, const bf16* __restrict__ X,
         }
 }
 
+// Halo-tile forward convolution for volumes whose sides are multiples of
+// (4, 8, 16). A block of 8 waves owns a run of units (output tile of 4 x 8 x 16
+// positions, 32 output channels); per stage (unit, 32-channel input chunk) it
+// holds in LDS the 6 x 10 x 18 halo of the tile and the chunk's weights for all
+// 27 taps, so every tap reads its A rows from LDS instead of HBM/L2. The next
+// stage's halo and weights are loaded into registers while the current one
+// computes. LDS rows are 64 bytes (32 channels) whose four 16-byte quads are
+// XOR-swizzled by (row >> 2) & 3, so 16 consecutive rows read by ds_read_b128
+// touch 16 distinct bank groups. Wave w computes the 2 x 32 positions at tile
+// x = w / 2, y = 4 (w % 2) .. + 3, z = 0 .. 15 against the unit's 32 channels.
+constexpr int HT_X = 4, HT_Y = 8, HT_Z = 16;
+constexpr int HH_X = HT_X + 2, HH_Y = HT_Y + 2, HH_Z = HT_Z + 2;
+constexpr int H_ROWS = HH_X * HH_Y * HH_Z;      // 1080 halo positions
+constexpr int HW_ROWS = 27 * 32;                // weights: (tap, co) rows
+constexpr int HC_THREADS = 512;
+constexpr int H_LOADS = (H_ROWS * 4 + HC_THREADS - 1) / HC_THREADS;
+constexpr int HW_LOADS = (HW_ROWS * 4 + HC_THREADS - 1) / HC_THREADS;
+constexpr int HC_LDS_BYTES = (H_ROWS + HW_ROWS) * 64;
+
+__device__ __forceinline__ int hswz(int row, int q) { return row * 64 + ((q ^ ((row >> 2) & 3)) << 4); }
+
+template <bool RELU>
+__global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __restrict__ X,
+                                                          const bf16* __restrict__ W, const float* __restrict__ bias,
+                                                          bf16* __restrict__ Y, int units, int upb) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    unsigned char* sH = lds;
+    unsigned char* sW = lds + H_ROWS * 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ntx = g.s0 / HT_X, nty = g.s1 / HT_Y, ntz = g.s2 / HT_Z;
+    const int nco = g.cout / 32, nch = g.cin / 32;
+    const int u0 = blockIdx.x * upb, u1 = min(u0 + upb, units);
+    if (u0 >= u1) return;
+    const int nstage = (u1 - u0) * nch;
+    const int plane = g.s1 * g.s2;
+    uint4 rh[H_LOADS], rw[HW_LOADS];
+    // global loads of stage s into registers
+    auto load = [&](int s) {
+        const int u = u0 + s / nch, ch = s % nch;
+        const int cob = u % nco;
+        int t = u / nco;
+        const int tz = t % ntz;
+        t /= ntz;
+        const int ty = t % nty;
+        t /= nty;
+        const int tx = t % ntx, n = t / ntx;
+        const int x0 = tx * HT_X - 1, y0 = ty * HT_Y - 1, z0 = tz * HT_Z - 1;
+        const long long vbase = (long long)n * g.s0 * plane;
+#pragma unroll
+        for (int i = 0; i < H_LOADS; ++i) {
+            const int idx = tid + i * HC_THREADS;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (idx < H_ROWS * 4) {
+                const int row = idx >> 2, q = idx & 3;
+                const int hz = row % HH_Z, hy = (row / HH_Z) % HH_Y, hx = row / (HH_Z * HH_Y);
+                const int gx = x0 + hx;
+                int gy = y0 + hy, gz = z0 + hz;
+                gy = gy < 0 ? gy + g.s1 : (gy >= g.s1 ? gy - g.s1 : gy);
+                gz = gz < 0 ? gz + g.s2 : (gz >= g.s2 ? gz - g.s2 : gz);
+                if (gx >= 0 && gx < g.s0)
+                    v = *reinterpret_cast<const uint4*>(
+                        X + (vbase + (long long)gx * plane + gy * g.s2 + gz) * g.cin + ch * 32 + q * 8);
+            }
+            rh[i] = v;
+        }
+#pragma unroll
+        for (int i = 0; i < HW_LOADS; ++i) {
+            const int idx = tid + i * HC_THREADS;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (idx < HW_ROWS * 4) {
+                const int row = idx >> 2, q = idx & 3;
+                const int tap = row >> 5, co = row & 31;
+                v = *reinterpret_cast<const uint4*>(W + ((long long)tap * g.cout + cob * 32 + co) * g.cin + ch * 32 +
+                                                    q * 8);
+            }
+            rw[i] = v;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int i = 0; i < H_LOADS; ++i) {
+            const int idx = tid + i * HC_THREADS;
+            if (idx < H_ROWS * 4) *reinterpret_cast<uint4*>(sH + hswz(idx >> 2, idx & 3)) = rh[i];
+        }
+#pragma unroll
+        for (int i = 0; i < HW_LOADS; ++i) {
+            const int idx = tid + i * HC_THREADS;
+            if (idx < HW_ROWS * 4) *reinterpret_cast<uint4*>(sW + hswz(idx >> 2, idx & 3)) = rw[i];
+        }
+    };
+    // this lane's rows: position r = lane % 32 of the wave's two row-blocks
+    const int wx = wave >> 1, wy = 4 * (wave & 1);
+    const int r = lane & 31, hq = lane >> 5;
+    const int hrow0 = (wx * HH_Y + wy + (r >> 4)) * HH_Z + (r & 15);   // row-block 0, tap (0, 0, 0)
+    f32x16 acc0 = f32x16{}, acc1 = f32x16{};
+    load(0);
+    store();
+    __syncthreads();
+    for (int s = 0; s < nstage; ++s) {
+        const bool more = s + 1 < nstage;
+        if (more) load(s + 1);
+#pragma unroll 3
+        for (int tap = 0; tap < 27; ++tap) {
+            const int a = tap / 9, b = (tap / 3) % 3, c = tap % 3;
+            const int hr = hrow0 + (a * HH_Y + b) * HH_Z + c;
+            const int wr = tap * 32 + r;
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk) {
+                const int q = kk * 2 + hq;
+                const bf16x8 bw = as_bf16x8(*reinterpret_cast<const uint4*>(sW + hswz(wr, q)));
+                const bf16x8 a0 = as_bf16x8(*reinterpret_cast<const uint4*>(sH + hswz(hr, q)));
+                const bf16x8 a1 = as_bf16x8(*reinterpret_cast<const uint4*>(sH + hswz(hr + 2 * HH_Z, q)));
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bw, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bw, acc1, 0, 0, 0);
+            }
+        }
+        if (s % nch == nch - 1) {
+            const int u = u0 + s / nch;
+            const int cob = u % nco;
+            int t = u / nco;
+            const int tz = t % ntz;
+            t /= ntz;
+            const int ty = t % nty;
+            t /= nty;
+            const int tx = t % ntx, n = t / ntx;
+            const int co = cob * 32 + r;
+            const float bv = bias ? bias[co] : 0.f;
+            const long long vbase = (long long)n * g.s0 * plane;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int pr = (e >> 2) * 8 + hq * 4 + (e & 3);   // row within the row-block
+                const int y = ty * HT_Y + wy + (pr >> 4), z = tz * HT_Z + (pr & 15), x = tx * HT_X + wx;
+                const long long p0 = vbase + (long long)x * plane + y * g.s2 + z;
+                float v0 = acc0[e] + bv, v1 = acc1[e] + bv;
+                if (RELU) {
+                    v0 = v0 > 0.f ? v0 : 0.f;
+                    v1 = v1 > 0.f ? v1 : 0.f;
+                }
+                Y[p0 * g.cout + co] = (bf16)v0;
+                Y[(p0 + 2 * g.s2) * g.cout + co] = (bf16)v1;
+            }
+            acc0 = f32x16{};
+            acc1 = f32x16{};
+        }
+        __syncthreads();
+        if (more) store();
+        __syncthreads();
+    }
+}
+
 // Weight gradient: dW[tap][co][ci] += sum_p dY[p][co] X[p + off(tap)][ci] in FP32
 // (atomics). Block: one tap, a 32 x 32 (co, ci) tile and a chunk of positions;
 // its 4 waves take interleaved 32-position stages of the chunk. Per stage a wave

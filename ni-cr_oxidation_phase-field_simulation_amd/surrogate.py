@@ -19,6 +19,30 @@ _FPTR = ctypes.POINTER(ctypes.c_float)
 _DPTR = ctypes.POINTER(ctypes.c_double)
 
 
+class UNetNorm(ctypes.Structure):
+    """nicr_unet_norm: fixed scales of phi, gb, the volume-statistics channels, and output gains."""
+    _fields_ = [("phi_mean", ctypes.c_float), ("phi_scale", ctypes.c_float), ("gb_mean", ctypes.c_float),
+                ("gb_scale", ctypes.c_float), ("c_mean", ctypes.c_float * 4), ("c_scale", ctypes.c_float * 4),
+                ("gain", ctypes.c_float * 3), ("jump", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {"phi_mean": self.phi_mean, "phi_scale": self.phi_scale, "gb_mean": self.gb_mean,
+                "gb_scale": self.gb_scale, "c_mean": list(self.c_mean), "c_scale": list(self.c_scale),
+                "gain": list(self.gain), "jump": self.jump}
+
+
+def make_norm(phi_mean=0.0, phi_scale=1.0, gb_mean=0.0, gb_scale=1.0, c_mean=(0.0,) * 4, c_scale=(1.0,) * 4,
+              gain=(1.0,) * 3, jump=50) -> UNetNorm:
+    nm = UNetNorm()
+    nm.jump = int(jump)
+    nm.phi_mean, nm.phi_scale, nm.gb_mean, nm.gb_scale = phi_mean, phi_scale, gb_mean, gb_scale
+    for k in range(4):
+        nm.c_mean[k], nm.c_scale[k] = c_mean[k], c_scale[k]
+    for k in range(3):
+        nm.gain[k] = gain[k]
+    return nm
+
+
 class SurOptions(ctypes.Structure):
     _fields_ = [("jump", ctypes.c_int32), ("conserve_mass", ctypes.c_int32), ("residual_tol", ctypes.c_double),
                 ("min_value", ctypes.c_double), ("max_value", ctypes.c_double)]
@@ -43,8 +67,8 @@ def _declare(lib):
         "nicr_unet_get_params": [_VOIDP, _FPTR],
         "nicr_unet_set_params": [_VOIDP, _FPTR],
         "nicr_unet_get_grads": [_VOIDP, _FPTR],
-        "nicr_unet_set_norm": [_VOIDP, _FPTR, _FPTR, _FPTR],
-        "nicr_unet_get_norm": [_VOIDP, _FPTR, _FPTR, _FPTR],
+        "nicr_unet_set_norm": [_VOIDP, ctypes.POINTER(UNetNorm)],
+        "nicr_unet_get_norm": [_VOIDP, ctypes.POINTER(UNetNorm)],
         "nicr_unet_forward": [_VOIDP, i, _FPTR, _FPTR],
         "nicr_unet_loss_backward": [_VOIDP, i, _FPTR, _FPTR, _DPTR],
         "nicr_unet_set_timing": [_VOIDP, i],
@@ -148,17 +172,19 @@ class UNet:
         _check(self.lib, self.lib.nicr_unet_get_grads(self.h, _fp(out)))
         return out
 
-    def set_norm(self, in_mean, in_scale, out_scale) -> None:
-        a, b, c = _f32(in_mean), _f32(in_scale), _f32(out_scale)
-        _check(self.lib, self.lib.nicr_unet_set_norm(self.h, _fp(a), _fp(b), _fp(c)))
+    def set_norm(self, norm: UNetNorm = None, **kw) -> None:
+        """Set the normalisation (a UNetNorm, or make_norm's keywords)."""
+        nm = norm if norm is not None else make_norm(**kw)
+        _check(self.lib, self.lib.nicr_unet_set_norm(self.h, ctypes.byref(nm)))
 
-    def norm(self):
-        a, b, c = np.empty(4, np.float32), np.empty(4, np.float32), np.empty(3, np.float32)
-        _check(self.lib, self.lib.nicr_unet_get_norm(self.h, _fp(a), _fp(b), _fp(c)))
-        return a, b, c
+    def norm(self) -> UNetNorm:
+        nm = UNetNorm()
+        _check(self.lib, self.lib.nicr_unet_get_norm(self.h, ctypes.byref(nm)))
+        return nm
 
     def forward(self, x):
-        """x: (nb, 4, s0, s1, s2) float32 -> (nb, 3, s0, s1, s2) normalised changes."""
+        """x: (nb, 7, s0, s1, s2) float32 (O, Cr, phi, gb, plain updates of O, Cr, phi)
+        -> (nb, 3, s0, s1, s2) normalised corrections."""
         x = _f32(x)
         out = np.empty((x.shape[0], 3) + self.shape, np.float32)
         _check(self.lib, self.lib.nicr_unet_forward(self.h, x.shape[0], _fp(x), _fp(out)))

@@ -52,6 +52,11 @@ CONV_CASES = [
     (1, 2, 4, 4, 384, 128),
     (2, 2, 2, 2, 256, 256),   # bottleneck of a 16^3 volume
     (1, 3, 1, 1, 32, 64),     # a single Y/Z cell: every wrapped tap is the cell itself
+    # sides multiples of (4, 8, 16): the halo-tile kernel
+    (1, 4, 8, 16, 32, 32),    # one tile: every halo row wraps in Y and Z, X zero padded
+    (2, 8, 8, 16, 96, 64),    # several input chunks, two output-channel units
+    (1, 4, 16, 32, 64, 32),
+    (1, 8, 8, 32, 384, 128),
 ]
 
 
@@ -121,14 +126,16 @@ def test_unet_forward_and_gradients_vs_torch(shape, nb):
     p = _random_net_params(net, 3)
     net.set_params(p)
     rng = np.random.default_rng(5)
-    fields = np.stack([rng.uniform(0, 2e-3, size=(nb,) + shape), 0.2 + 0.01 * rng.normal(size=(nb,) + shape),
-                       rng.uniform(0, 1, size=(nb,) + shape), rng.uniform(0, 1, size=(nb,) + shape)], axis=1)
-    mean = np.array([1e-3, 0.2, 0.5, 0.5], np.float32)
-    scale = np.array([6e-4, 0.01, 0.3, 0.3], np.float32)
-    net.set_norm(mean, scale, np.ones(3, np.float32))
+    sz = (nb,) + shape
+    fields = np.stack([rng.uniform(0, 2e-3, size=sz), 0.2 + 0.01 * rng.normal(size=sz), rng.uniform(0, 1, size=sz),
+                       rng.uniform(0, 1, size=sz), 1e-7 * rng.normal(size=sz), 1e-6 * rng.normal(size=sz),
+                       1e-4 * rng.normal(size=sz)], axis=1)
+    norm = dict(phi_mean=0.5, phi_scale=0.3, gb_mean=0.5, gb_scale=0.3, c_mean=[-3.5, 0.2, -7.0, -6.0],
+                c_scale=[0.5, 0.02, 0.5, 0.5], gain=[1.0, 1.0, 1.0], jump=50)
+    net.set_norm(**norm)
     target = rng.normal(size=(nb, 3) + shape).astype(np.float32)
     out = net.forward(fields)
-    ref_out, ref_loss, ref_g = R.loss_and_grads(net.params(), fields, target, mean, scale)
+    ref_out, ref_loss, ref_g = R.loss_and_grads(net.params(), fields, target, norm)
     err = np.linalg.norm(out - ref_out) / np.linalg.norm(ref_out)
     assert err < 1e-2, f"forward rel L2 {err:.3e}"
     loss = net.loss_backward(fields, target)
@@ -164,14 +171,15 @@ def _closed_model(nicr, shape, seed, **kw):
 
 
 def test_rejected_jumps_equal_plain_steps(nicr):
-    """An untrained network (zero head: the proposal does not move) is rejected
-    at every jump; the model then equals plain stepping bitwise."""
+    """With a residual tolerance no proposal meets, every jump is rejected and
+    the model equals plain stepping bitwise (fields and budgets)."""
     shape = (16, 16, 24)
     a = _closed_model(nicr, shape, 4)
     b = _closed_model(nicr, shape, 4)
     net = _sur().UNet(shape, seed=1)
-    net.set_norm([1e-3, 0.2, 0.0, 0.5], [2e-4, 3e-3, 1.0, 0.3], [1e-6, 1e-5, 0.0])
-    rep = net.advance(a, 130, jump=20, residual_tol=0.2)
+    net.set_norm(gb_mean=0.5, gb_scale=0.3, c_mean=[-3.5, 0.2, -7.0, -6.0], c_scale=[0.5, 0.02, 0.5, 0.5],
+                 gain=[0.05, 0.05, 0.0], jump=20)
+    rep = net.advance(a, 130, jump=20, residual_tol=1e-12)
     assert rep.status == 0 and rep.steps_done == 130
     assert rep.jumps_accepted == 0 and rep.jumps_rejected >= 5
     assert rep.plain_steps == 130
@@ -195,9 +203,9 @@ def test_training_reduces_loss_and_jumps_are_checked(nicr):
             rep = net.record(m, 20)
             assert rep.status == 0
     assert net.samples == 12
+    # a recorded sample consumes 1 + jump steps of its trajectory
     net.fit_norm()
-    mean, scale, out_scale = net.norm()
-    assert out_scale[2] == 0.0     # phi never changes in this workload
+    assert net.norm().gain[2] == 0.0     # phi never changes in this workload
     losses = net.train(300, batch=2, lr=1e-3, seed=1)
     first, last = float(np.mean(losses[:10])), float(np.mean(losses[-30:]))
     print(f"training loss {first:.4f} -> {last:.4f}")

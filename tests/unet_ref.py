@@ -90,25 +90,49 @@ def forward(params, x0):
     return torch.einsum("ncxyz,oc->noxyz", d1, hw) + hb.view(1, 3, 1, 1, 1)
 
 
-def make_input(fields, mean, scale):
-    """fields (nb, 4, s0, s1, s2) float32 -> x0 (nb, 32, ...) as the device's input kernel forms it."""
+def volume_stats(f):
+    """Statistics of one volume (binary64): mean and population std of O and Cr,
+    RMS of the updates of O, Cr, phi."""
+    out = []
+    for c in (0, 1):
+        a = np.asarray(f[c], np.float64)
+        m = a.sum() / a.size
+        out += [m, np.sqrt(max(0.0, (a * a).sum() / a.size - m * m))]
+    for c in (4, 5, 6):
+        a = np.asarray(f[c], np.float64)
+        out.append(np.sqrt((a * a).sum() / a.size))
+    return out
+
+
+def make_input(fields, norm):
+    """fields (nb, 7, s0, s1, s2) float32 -> x0 (nb, 32, ...) as the device's input kernel forms it.
+
+    norm: dict with phi_mean, phi_scale, gb_mean, gb_scale, c_mean[4], c_scale[4]."""
     f = np.asarray(fields, np.float32)
-    inv = (np.float32(1.0) / np.asarray(scale, np.float32)).astype(np.float32)
-    x = (f - np.asarray(mean, np.float32).reshape(1, 4, 1, 1, 1)) * inv.reshape(1, 4, 1, 1, 1)
-    x = bf16(x.astype(np.float32))
     x0 = np.zeros((f.shape[0], 32) + f.shape[2:], np.float32)
-    x0[:, :4] = x
-    return x0
+    for b in range(f.shape[0]):
+        mo, so, mc, sc, ro, rc, rp = volume_stats(f[b])
+        mean = [mo, mc, norm["phi_mean"], norm["gb_mean"]]
+        inv = [1.0 / so if so > 0 else 0.0, 1.0 / sc if sc > 0 else 0.0, 1.0 / norm["phi_scale"],
+               1.0 / norm["gb_scale"]]
+        for c in range(4):
+            x0[b, c] = ((f[b, c].astype(np.float64) - mean[c]) * inv[c]).astype(np.float32)
+        for k, r in enumerate((ro, rc, rp)):
+            x0[b, 4 + k] = (f[b, 4 + k].astype(np.float64) * (1.0 / r if r > 0 else 0.0)).astype(np.float32)
+        cst = [np.log10(max(so, 1e-300)), mc, np.log10(max(ro, 1e-300)), np.log10(max(rc, 1e-300))]
+        for k in range(4):
+            x0[b, 7 + k] = np.float32((cst[k] - norm["c_mean"][k]) / norm["c_scale"][k])
+    return bf16(x0)
 
 
-def loss_and_grads(pvec, fields, target, mean, scale):
+def loss_and_grads(pvec, fields, target, norm):
     """(outputs, L, flat gradient) of L = (1/3) sum (out - target)^2 for the flat parameters pvec."""
     convs, (hw, hb) = split_params(pvec)
     tp = []
     for w, b in convs:
         tp.append((torch.tensor(bf16(w), requires_grad=True), torch.tensor(b, requires_grad=True)))
     th = (torch.tensor(hw, requires_grad=True), torch.tensor(hb, requires_grad=True))
-    x0 = torch.from_numpy(make_input(fields, mean, scale))
+    x0 = torch.from_numpy(make_input(fields, norm))
     out = forward((tp, th), x0)
     t = torch.from_numpy(np.asarray(target, np.float32))
     loss = ((out - t) ** 2).sum() / 3.0
