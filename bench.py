@@ -186,6 +186,100 @@ def ensemble_bench(nicr, n_inst, steps, seed):
             "speedup_vs_sequential": round((t5 - t4) / (t1 - t0), 3)}
 
 
+def unet_flops_per_cell():
+    """Multiply-add flops of one U-Net forward pass per cell of the volume (the
+    3x3x3 convolutions; (cin, cout) per level from include/nicr_nn.h)."""
+    levels = {0: [(32, 32), (32, 32), (96, 32), (32, 32)], 1: [(32, 64), (64, 64), (192, 64), (64, 64)],
+              2: [(64, 128), (128, 128), (384, 128), (128, 128)], 3: [(128, 256), (256, 256)]}
+    return sum(2 * 27 * ci * co / 8 ** lv for lv, convs in levels.items() for ci, co in convs)
+
+
+def surrogate_bench(nicr, n_train, n_run, n_traj, iters, cycles, tol, seed):
+    """Learned surrogate on the C2-like closed workload: training set from
+    n_traj plain binary64 trajectories at n_train^3, Adam on the GPU, then
+    n_run^3 from one state (after 200 plain steps) advanced cycles * 52 steps by
+    the plain binary64 step and by surrogate jumps of 50 steps; both timed, the
+    surrogate's final fields and O/Cr totals compared with the plain run's."""
+    sur = importlib.import_module(PKG_NAME + ".surrogate")
+    syn = importlib.import_module(PKG_NAME + ".synthetic")
+
+    def log(msg):
+        print(f"[bench] surrogate: {msg}", file=sys.stderr, flush=True)
+
+    J = 50
+    net = sur.UNet((n_train,) * 3, max_batch=2, train=True, max_samples=n_traj, seed=seed)
+    rec = sur.record_trajectories(net, nicr, n_traj, jump=J, seed=seed, log=log)
+    net.fit_norm()
+    tr = sur.train_schedule(net, iters, batch=2, lr=1e-3, seed=seed, log=log)
+    params, norm = net.params(), net.norm()
+    net.close()
+    shape = (n_run,) * 3
+    cells = n_run ** 3
+    t0 = time.perf_counter()
+    state = syn.closed_alloy(shape, n_seeds=16 * (n_run // n_train) ** 3, seed=seed + 7)
+    log(f"{n_run}^3 state built in {time.perf_counter() - t0:.1f} s")
+    plain = sur.closed_model(nicr, shape, state)
+    sm = sur.closed_model(nicr, shape, state)
+    del state
+    for m in (plain, sm):
+        r = m.step(200)
+        if r.steps_done != 200:
+            raise RuntimeError("surrogate workload: warm-up step rejected")
+    inet = sur.UNet(shape, seed=1)
+    inet.set_params(params)
+    inet.set_norm(norm)
+    inet.set_timing(True)
+    mass0 = [sum(sm.stats(f)[0]) for f in ("o", "cr")]
+    nsteps = cycles * (J + 2)
+    plain.sync()
+    t0 = time.perf_counter()
+    r = plain.step(nsteps)
+    plain.sync()
+    t_plain = time.perf_counter() - t0
+    sm.sync()
+    t0 = time.perf_counter()
+    rep = inet.advance(sm, nsteps, jump=J, residual_tol=tol)
+    sm.sync()
+    t_sur = time.perf_counter() - t0
+    if r.steps_done != nsteps or rep.steps_done != nsteps or rep.status != 0:
+        raise RuntimeError("surrogate workload: a plain step was rejected")
+    err = {}
+    for f in ("o", "cr", "p"):
+        a, b = sm.get(f), plain.get(f)
+        nb = np.linalg.norm(b)
+        err[f] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.abs(a - b).max())
+    mass_sur = [sum(sm.stats(f)[0]) for f in ("o", "cr")]
+    mass_plain = [sum(plain.stats(f)[0]) for f in ("o", "cr")]
+    fwd_ms = inet.forward_ms()
+    flops = unet_flops_per_cell() * cells
+    dt = plain.c.dt
+    out = {
+        "workload": f"3D {n_run}^3 closed, {16 * (n_run // n_train) ** 3} static Voronoi grains (GB mask), "
+                    f"Cr 0.20+-0.005, O 1e-3 (+-20%), no oxide; binary64 steps; jump {J} steps",
+        "speedup": round(t_plain / t_sur, 3),
+        "sim_time_per_s_plain_fp64": nsteps * dt / t_plain,
+        "sim_time_per_s_surrogate": nsteps * dt / t_sur,
+        "steps": nsteps, "ms_plain": round(t_plain * 1e3, 2), "ms_surrogate": round(t_sur * 1e3, 2),
+        "jumps_accepted": rep.jumps_accepted, "jumps_rejected": rep.jumps_rejected, "plain_steps": rep.plain_steps,
+        "residual_tol": tol, "max_residual": [round(rep.max_residual_o, 4), round(rep.max_residual_cr, 4)],
+        "rel_l2_vs_plain": {k: float(f"{v:.3e}") for k, v in err.items()},
+        "mass_rel_vs_plain": [float(f"{abs(a - b) / abs(b):.3e}") for a, b in zip(mass_sur, mass_plain)],
+        "mass_rel_vs_start": [float(f"{abs(a - b) / abs(b):.3e}") for a, b in zip(mass_sur, mass0)],
+        "forward_ms": round(fwd_ms, 3),
+        "forward_roofline": {"bound": "mfma", "achieved": round(flops / (fwd_ms * 1e-3) / 1e12, 1), "peak": 2500.0,
+                             "unit": "TFLOP/s", "frac": round(flops / (fwd_ms * 1e-3) / 1e12 / 2500.0, 4),
+                             "flops_per_forward": flops},
+        "training": {"volume": f"{n_train}^3", "trajectories": rec["trajectories"],
+                     "plain_steps": rec["plain_steps"], "record_s": round(rec["seconds"], 1),
+                     "iters": tr["iters"], "batch": tr["batch"], "train_s": round(tr["seconds"], 1),
+                     "loss_first": round(tr["loss_first"], 4), "loss_last": round(tr["loss_last"], 4)},
+    }
+    inet.close()
+    plain.close()
+    sm.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +292,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 2D ensemble measurement")
     ap.add_argument("--ensemble-only", action="store_true", help="only the 2D ensemble measurement")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate measurement")
+    ap.add_argument("--surrogate-only", action="store_true", help="only the learned-surrogate measurement")
+    ap.add_argument("--sur-traj", type=int, default=200, help="training trajectories of the surrogate")
+    ap.add_argument("--sur-iters", type=int, default=2000, help="Adam iterations of the surrogate")
+    ap.add_argument("--sur-train-n", type=int, default=128, help="side of the training volumes")
+    ap.add_argument("--sur-run-n", type=int, default=256, help="side of the surrogate run")
+    ap.add_argument("--sur-cycles", type=int, default=20, help="jump cycles (52 steps each) of the timed run")
+    ap.add_argument("--sur-tol", type=float, default=0.1, help="trapezoid residual tolerance of a jump")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the 1 GiB device copy reference (rocprof counter calibration)")
     args = ap.parse_args()
@@ -211,6 +313,13 @@ def main():
         m = nicr.Model(cfg)
         ms = m.copy_probe(1 << 30, 20)
         print(json.dumps({"copy_probe_bytes": 1 << 30, "ms": ms, "GBs_read_plus_write": 2 * (1 << 30) / ms / 1e6}))
+        return
+    def run_surrogate():
+        return surrogate_bench(nicr, args.sur_train_n, args.sur_run_n, args.sur_traj, args.sur_iters,
+                               args.sur_cycles, args.sur_tol, 20260909)
+
+    if args.surrogate_only:
+        print(json.dumps({"surrogate": run_surrogate()}), flush=True)
         return
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_bench(nicr, 64, 200, 20260909)}), flush=True)
@@ -251,7 +360,7 @@ def main():
     step_bytes = cells * (48 + 8 * args.eta)   # FP64 o/cr/phi + FP32 eta, read + write
     traffic = measured_traffic("k_eta_march")
     out = {
-        "metric": "Gcell-steps/s @256^3 mixed vs FP64 GPU, % HBM peak",
+        "metric": "Gcell-steps/s @256^3 mixed vs FP64 GPU, % HBM peak; surrogate speedup @1e-3 L2",
         "value": round(value, 4),
         "unit": "Gcell-steps/s",
         "n_gpus": 1,
@@ -282,6 +391,8 @@ def main():
     }
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(nicr, 64, 200, 20260909)
+    if not args.no_surrogate:
+        out["surrogate"] = run_surrogate()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nicr, args.eta, 20260909)
     print(json.dumps(out), flush=True)

@@ -9,6 +9,7 @@ reference. There is no CPU path: every call runs the library's kernels.
 from __future__ import annotations
 
 import ctypes
+import time
 
 import numpy as np
 
@@ -250,3 +251,71 @@ class UNet:
             self.close()
         except Exception:
             pass
+
+
+# ---- training pipeline on trajectories of the plain step ----
+
+def closed_model(nicr, shape, state, precision="fp64"):
+    """A binary64 3D Model of the surrogate workload (static grains, closed
+    boundaries, no Ckp correction, no insertion) holding `state` = (o, cr, p, gb)."""
+    cfg = nicr.make_config(3, nx=shape[0], ny=shape[1], nz=shape[2], grains=True, grain_evolution=False,
+                           bc="closed", ck="off", bounds="legacy", nucleation="off", precision=precision)
+    m = nicr.Model(cfg)
+    for k, a in zip(("o", "cr", "p", "gb"), state):
+        m.set(k, a)
+    return m
+
+
+def record_trajectories(net, nicr, n_traj, jump=50, n_struct=16, n_seeds=16, warm=(20, 600), seed=0, log=None):
+    """One training sample from each of n_traj plain binary64 trajectories.
+
+    Trajectory t starts from the grain structure t % n_struct (Voronoi, n_seeds
+    seeds), Cr mean U(0.17, 0.23) with U(-0.005, 0.005) noise, O background
+    10^U(-3.5, -2.5) with 20 % noise, takes a random number of plain steps in
+    `warm`, then records (state, plain update, change over `jump` steps)."""
+    from . import synthetic as syn
+    shape = net.shape
+    t0 = time.perf_counter()
+    gbs = []
+    for s in range(n_struct):
+        _, dist, _ = syn.voronoi_labels(shape, n_seeds, seed=seed + 100 + s)
+        gbs.append(syn.gb_mask_from_distance(dist))
+    t1 = time.perf_counter()
+    steps = 0
+    for t in range(n_traj):
+        r = np.random.default_rng(seed + 1000 + t)
+        crm, ob = r.uniform(0.17, 0.23), 10 ** r.uniform(-3.5, -2.5)
+        cr = crm + r.uniform(-0.005, 0.005, size=shape)
+        o = ob * (1.0 + r.uniform(-0.2, 0.2, size=shape))
+        m = closed_model(nicr, shape, (o, cr, np.zeros(shape), gbs[t % n_struct]))
+        w = int(r.integers(warm[0], warm[1]))
+        rep = m.step(w)
+        rr = net.record(m, jump)
+        if rr.status != 0:
+            raise RuntimeError(f"trajectory {t}: a plain step was rejected (status {rr.status})")
+        steps += w + 1 + jump
+        m.close()
+    t2 = time.perf_counter()
+    if log:
+        log(f"{n_struct} grain structures in {t1 - t0:.1f} s; {n_traj} trajectories ({steps} plain steps) "
+            f"recorded in {t2 - t1:.1f} s")
+    return {"trajectories": n_traj, "plain_steps": steps, "seconds": t2 - t0}
+
+
+def train_schedule(net, iters, batch=2, lr=1e-3, seed=1, log=None):
+    """Adam with the learning rate stepped down to 0.3x at 60 % and 0.1x at 85 % of iters."""
+    t0 = time.perf_counter()
+    done, losses = 0, []
+    b1, b2 = int(0.6 * iters), int(0.85 * iters)
+    while done < iters:
+        nxt = b1 if done < b1 else (b2 if done < b2 else iters)
+        cur = lr if done < b1 else (0.3 * lr if done < b2 else 0.1 * lr)
+        k = min(250, nxt - done)
+        ls = net.train(k, batch=batch, lr=cur, seed=seed + done)
+        losses.append(ls)
+        done += k
+        if log:
+            log(f"iter {done}: loss {float(np.mean(ls[-min(50, len(ls)):])):.4f} ({time.perf_counter() - t0:.1f} s)")
+    allv = np.concatenate(losses)
+    return {"iters": iters, "batch": batch, "seconds": time.perf_counter() - t0,
+            "loss_first": float(np.mean(allv[:20])), "loss_last": float(np.mean(allv[-100:]))}

@@ -59,6 +59,27 @@ def voronoi_labels(shape, n_seeds, seed=20260909):
     rng = np.random.default_rng(seed)
     dims = len(shape)
     seeds = np.stack([rng.uniform(0, n, n_seeds) for n in shape], axis=1)
+    if n_seeds >= 2:
+        try:
+            from scipy.spatial import cKDTree
+        except ImportError:
+            cKDTree = None
+        if cKDTree is not None:
+            # k-d tree over the seeds with periodic Y (and Z); X is made
+            # non-periodic by a box far longer than every in-mesh distance
+            box = np.array([4.0 * sum(shape)] + [float(n) for n in shape[1:]])
+            tree = cKDTree(seeds, boxsize=box)
+            labels = np.empty(shape, dtype=np.int32)
+            dist = np.empty(shape)
+            axes = [np.arange(n) + 0.5 for n in shape[1:]]
+            for x0 in range(0, shape[0], 16):
+                xs = np.arange(x0, min(shape[0], x0 + 16)) + 0.5
+                pts = np.stack(np.meshgrid(xs, *axes, indexing="ij"), axis=-1).reshape(-1, dims)
+                d, i = tree.query(pts, k=2, workers=-1)
+                sl = (slice(x0, x0 + len(xs)),)
+                labels[sl] = i[:, 0].reshape((len(xs),) + tuple(shape[1:]))
+                dist[sl] = (0.5 * (d[:, 1] - d[:, 0])).reshape((len(xs),) + tuple(shape[1:]))
+            return labels, dist, seeds
     labels = np.empty(shape, dtype=np.int32)
     dist = np.empty(shape)
     for x0 in range(shape[0]):
