@@ -193,6 +193,131 @@ __global__ void k_apply_jump(int V, const float* __restrict__ out, VolNorm vn, c
     ph[p] += vn.jump * up[p] + (double)v.z * vn.out_scale[2];
 }
 
+// ---- fused passes of a surrogate jump (binary64 state, block partial sums) ----
+
+constexpr int SUR_BLOCKS = 1024;   // blocks of the fused passes; partial sums: SUR_BLOCKS x 8
+
+// block sum of NV binary64 values into part[blockIdx.x * 8 + k]
+template <int NV>
+__device__ __forceinline__ void block_sums8(double (&v)[NV], double* __restrict__ part) {
+    __shared__ double red[NV][4];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double x = v[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV)
+        part[blockIdx.x * 8 + threadIdx.x] =
+            red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+}
+
+// Before a jump: u = x0 - x_-1 for O, Cr, phi; backup of x0; sums of O, O^2,
+// Cr, Cr^2, u_O^2, u_Cr^2, u_phi^2.
+__global__ __launch_bounds__(256) void k_sur_pre(long long n, const double* __restrict__ o,
+                                                 const double* __restrict__ cr, const double* __restrict__ ph,
+                                                 const double* __restrict__ op, const double* __restrict__ crp,
+                                                 const double* __restrict__ php, double* __restrict__ u0,
+                                                 double* __restrict__ u1, double* __restrict__ u2,
+                                                 double* __restrict__ b0, double* __restrict__ b1,
+                                                 double* __restrict__ b2, double* __restrict__ part) {
+    double v[7] = {0., 0., 0., 0., 0., 0., 0.};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double a = o[i], c = cr[i], p = ph[i];
+        const double d0 = a - op[i], d1 = c - crp[i], d2 = p - php[i];
+        u0[i] = d0;
+        u1[i] = d1;
+        u2[i] = d2;
+        b0[i] = a;
+        b1[i] = c;
+        b2[i] = p;
+        v[0] += a;
+        v[1] += a * a;
+        v[2] += c;
+        v[3] += c * c;
+        v[4] += d0 * d0;
+        v[5] += d1 * d1;
+        v[6] += d2 * d2;
+    }
+    block_sums8<7>(v, part);
+}
+
+// The proposal x0 + J u + out * out_scale written to the state; sums of the
+// new O and Cr and min/max of O, Cr, phi (max as min of the negation).
+__global__ __launch_bounds__(256) void k_sur_apply(long long n, const float* __restrict__ out, VolNorm vn,
+                                                   const double* __restrict__ u0, const double* __restrict__ u1,
+                                                   const double* __restrict__ u2, double* __restrict__ o,
+                                                   double* __restrict__ cr, double* __restrict__ ph,
+                                                   double* __restrict__ part) {
+    double s[2] = {0., 0.};
+    double mn[6] = {INFINITY, INFINITY, INFINITY, INFINITY, INFINITY, INFINITY};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float4 w = *reinterpret_cast<const float4*>(out + 4 * i);
+        const double a = o[i] + vn.jump * u0[i] + (double)w.x * vn.out_scale[0];
+        const double c = cr[i] + vn.jump * u1[i] + (double)w.y * vn.out_scale[1];
+        const double p = ph[i] + vn.jump * u2[i] + (double)w.z * vn.out_scale[2];
+        o[i] = a;
+        cr[i] = c;
+        ph[i] = p;
+        s[0] += a;
+        s[1] += c;
+        mn[0] = fmin(mn[0], a);
+        mn[1] = fmin(mn[1], -a);
+        mn[2] = fmin(mn[2], c);
+        mn[3] = fmin(mn[3], -c);
+        mn[4] = fmin(mn[4], p);
+        mn[5] = fmin(mn[5], -p);
+    }
+    // non-finite values poison the sums and show in the bounds test on the host
+    __shared__ double red[8][4];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        double x = k < 2 ? s[k] : mn[k - 2];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double y = __shfl_xor(x, off, 64);
+            x = k < 2 ? x + y : fmin(x, y);
+        }
+        if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int k = threadIdx.x;
+        const double* r = red[k];
+        part[blockIdx.x * 8 + k] = k < 2 ? r[0] + r[1] + r[2] + r[3] : fmin(fmin(r[0], r[1]), fmin(r[2], r[3]));
+    }
+}
+
+__global__ void k_scale2(long long n, double* __restrict__ a, double fa, double* __restrict__ b, double fb) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        a[i] *= fa;
+        b[i] *= fb;
+    }
+}
+
+// Trapezoid residuals of O and Cr: R = (xh - x0) - (J/2)(u + (xh1 - xh));
+// sums of R^2 and (xh - x0)^2 for each.
+__global__ __launch_bounds__(256) void k_sur_trap(long long n, const double* __restrict__ ho,
+                                                  const double* __restrict__ h1o, const double* __restrict__ xo,
+                                                  const double* __restrict__ uo, const double* __restrict__ hc,
+                                                  const double* __restrict__ h1c, const double* __restrict__ xc,
+                                                  const double* __restrict__ uc, double halfJ,
+                                                  double* __restrict__ part) {
+    double v[4] = {0., 0., 0., 0.};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double d0 = ho[i] - xo[i], r0 = d0 - halfJ * (uo[i] + (h1o[i] - ho[i]));
+        const double d1 = hc[i] - xc[i], r1 = d1 - halfJ * (uc[i] + (h1c[i] - hc[i]));
+        v[0] += r0 * r0;
+        v[1] += d0 * d0;
+        v[2] += r1 * r1;
+        v[3] += d1 * d1;
+    }
+    block_sums8<4>(v, part);
+}
+
 // acc += sum (d - J u)^2 (float arrays; binary64 sums)
 __global__ __launch_bounds__(256) void k_corr_sq(long long n, const float* __restrict__ d, const float* __restrict__ u,
                                                  double J, double* __restrict__ acc) {
@@ -406,6 +531,8 @@ struct Net {
     // binary64 state backup of a surrogate jump (O, Cr, phi)
     double* bk[3] = {nullptr, nullptr, nullptr};
     double* du[3] = {nullptr, nullptr, nullptr};   // plain update of O, Cr, phi before a jump
+    double* part = nullptr;                          // block partial sums of the fused jump passes
+    std::vector<double> part_host;
     int ds_jump = 0;                                 // steps of the recorded changes
     long long bk_cells = 0;
     bool timing = false;
@@ -453,7 +580,7 @@ Geo geo(const Net* n, int nb, int level, int cin, int cout, int taps) {
 }
 
 // Halo-tile kernel settings: compute units of the current device (one
-// resident block each: 124 KB of LDS) and whether the kernel may be used
+// resident block each: 152 KB of LDS) and whether the kernel may be used
 // (NICR_CONV_HALO=0 keeps the per-tap kernel everywhere).
 struct HaloCfg {
     int ncu = 0;
@@ -821,9 +948,9 @@ int check_ctx_shape(Net* n, Ctx* c) {
 }
 
 // network input from the model's current state (binary64 O, Cr, phi, the
-// mask, and the plain update held in n->du); *vn receives the volume's
-// normalisation
-int input_from_ctx(Net* n, Ctx* c, int slot, VolNorm* vn) {
+// mask, and the plain update held in n->du) with the volume statistics st;
+// *vn receives the volume's normalisation
+int input_from_ctx(Net* n, Ctx* c, int slot, const double st[N_STAT], VolNorm* vn) {
     if (c->mask_dirty) {
         do_mask(c);
         c->mask_dirty = false;
@@ -831,9 +958,6 @@ int input_from_ctx(Net* n, Ctx* c, int slot, VolNorm* vn) {
     HIPCHK(hipStreamSynchronize(c->stream));
     const long long V = n->V(0);
     const int cur = c->cur;
-    double st[N_STAT];
-    int rc = device_stats(n, c->o[cur], c->cr[cur], n->du, V, st);
-    if (rc) return rc;
     *vn = vol_norm(n->norm, st);
     bf16* x0 = n->x0 + (long long)slot * V * 32;
     if (c->te == 8)
@@ -1316,14 +1440,6 @@ int nicr_sur_eval(void* h, int first, int count, double* mse) {
 
 // ---- surrogate-accelerated stepping ----
 
-static int sur_sum(Ctx* c, int field, double* s) {
-    nicr_stats st;
-    int rc = nicr_field_stats(c, field, &st);
-    if (rc) return rc;
-    *s = st.sum[0] + st.sum[1];
-    return NICR_OK;
-}
-
 static int sur_buffers(Net* n) {
     if (n->bk[0]) return NICR_OK;
     const long long V = n->V(0);
@@ -1332,19 +1448,24 @@ static int sur_buffers(Net* n) {
         if ((rc = nalloc_t(n, &n->bk[i], V))) return rc;
     for (int i = 0; i < N_OUT; ++i)
         if ((rc = nalloc_t(n, &n->du[i], V))) return rc;
+    if ((rc = nalloc_t(n, &n->part, (long long)SUR_BLOCKS * 8))) return rc;
+    n->part_host.resize((size_t)SUR_BLOCKS * 8);
     n->bk_cells = V;
     return NICR_OK;
 }
 
-// ||R|| / ||xh - x0|| of the trapezoid residual (infinite for a zero jump)
-static int sur_trap(Net* n, Ctx* c, const double* xh, const double* xh1, const double* x0, const double* u0, int J,
-                    double* ratio) {
-    double acc[2];
-    HIPCHK(hipMemsetAsync(n->dacc, 0, 2 * 8, n->stream));
-    hipLaunchKernelGGL(k_trap, dim3(1024), dim3(256), 0, n->stream, c->ncell, xh, xh1, x0, u0, 0.5 * J, n->dacc);
-    HIPCHK(hipMemcpyAsync(acc, n->dacc, 2 * 8, hipMemcpyDeviceToHost, n->stream));
+// block partial sums of a fused pass -> host (columns 0..nv-1); min columns
+// (from `first_min` on) are reduced by min instead of sum
+static int sur_partials(Net* n, int nv, long double* sum, int first_min = 8) {
+    HIPCHK(hipMemcpyAsync(n->part_host.data(), n->part, (size_t)SUR_BLOCKS * 8 * 8, hipMemcpyDeviceToHost, n->stream));
     HIPCHK(hipStreamSynchronize(n->stream));
-    *ratio = acc[1] > 0. ? std::sqrt(acc[0] / acc[1]) : INFINITY;
+    for (int k = 0; k < nv; ++k) sum[k] = k < first_min ? 0.L : (long double)INFINITY;
+    for (int b = 0; b < SUR_BLOCKS; ++b)
+        for (int k = 0; k < nv; ++k) {
+            const double v = n->part_host[(size_t)b * 8 + k];
+            if (k < first_min) sum[k] += (long double)v;
+            else if (v < sum[k] || std::isnan(v)) sum[k] = v;
+        }
     return NICR_OK;
 }
 
@@ -1381,30 +1502,31 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
             if ((rc = plain(left))) return rc;
             break;
         }
-        // the plain step before the jump: its update u0 enters the residual
+        // the plain step before the jump: its update u0 enters the input and the residual
         if ((rc = plain(1))) return rc;
         if (rep->status != NICR_STEP_OK) break;
-        const int cur = c->cur;
-        HIPCHK(hipStreamSynchronize(c->stream));
-        hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], c->o[cur ^ 1], n->du[0]);
-        hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[cur], c->cr[cur ^ 1], n->du[1]);
-        hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->p[cur], c->p[cur ^ 1], n->du[2]);
-        HIPCHK(hipMemcpyAsync(n->bk[0], c->o[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
-        HIPCHK(hipMemcpyAsync(n->bk[1], c->cr[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
-        HIPCHK(hipMemcpyAsync(n->bk[2], c->p[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
-        double m0[2] = {0., 0.};
-        if (conserve && ((rc = sur_sum(c, NICR_FIELD_O, &m0[0])) || (rc = sur_sum(c, NICR_FIELD_CR, &m0[1]))))
-            return rc;
         DevCtl saved;
-        if ((rc = sync_ctl(c, &saved))) return rc;
-        // proposal x0 + network(x0)
+        if ((rc = sync_ctl(c, &saved))) return rc;   // also: the step has finished
+        const int cur = c->cur;
+        // one pass: u0, backup of x0, volume statistics and O/Cr totals
+        hipLaunchKernelGGL(k_sur_pre, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, V, c->o[cur], c->cr[cur], c->p[cur],
+                           c->o[cur ^ 1], c->cr[cur ^ 1], c->p[cur ^ 1], n->du[0], n->du[1], n->du[2], n->bk[0],
+                           n->bk[1], n->bk[2], n->part);
+        long double ps[7];
+        if ((rc = sur_partials(n, 7, ps))) return rc;
+        const double m0[2] = {(double)ps[0], (double)ps[2]};
+        const double sums[N_STAT] = {(double)ps[0], (double)ps[1], (double)ps[2], (double)ps[3],
+                                     (double)ps[4], (double)ps[5], (double)ps[6]};
+        double st[N_STAT];
+        stat_from_sums(sums, (double)V, st);
+        // proposal x0 + J u0 + network correction, in place in the current buffers
         VolNorm vn;
-        if ((rc = input_from_ctx(n, c, 0, &vn))) return rc;
+        if ((rc = input_from_ctx(n, c, 0, st, &vn))) return rc;
         forward(n, 1);
-        hipLaunchKernelGGL(k_apply_jump, dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, n->out, vn, n->du[0],
-                           n->du[1], n->du[2], c->o[cur],
-                           c->cr[cur], c->p[cur]);
-        HIPCHK(hipStreamSynchronize(n->stream));
+        hipLaunchKernelGGL(k_sur_apply, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, V, n->out, vn, n->du[0], n->du[1],
+                           n->du[2], c->o[cur], c->cr[cur], c->p[cur], n->part);
+        long double pa[8];
+        if ((rc = sur_partials(n, 8, pa, 2))) return rc;
         if (n->timing) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, n->ev[0], n->ev[1]) == hipSuccess) {
@@ -1413,23 +1535,21 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
             }
         }
         c->refresh_valid = false;
-        bool ok = true;
-        nicr_stats so, scr, sp;
-        if ((rc = nicr_field_stats(c, NICR_FIELD_O, &so)) || (rc = nicr_field_stats(c, NICR_FIELD_CR, &scr)) ||
-            (rc = nicr_field_stats(c, NICR_FIELD_PHI, &sp)))
-            return rc;
+        // O and Cr back to their totals at x0 (closed boundaries conserve both)
+        double f[2] = {1., 1.};
+        const double s1[2] = {(double)pa[0], (double)pa[1]};
         if (conserve) {
-            const double s_o = so.sum[0] + so.sum[1], s_c = scr.sum[0] + scr.sum[1];
-            if (s_o > 0. && m0[0] > 0.)
-                hipLaunchKernelGGL(k_scale, dim3(nblk(V)), dim3(256), 0, c->stream, V, c->o[cur], m0[0] / s_o);
-            if (s_c > 0. && m0[1] > 0.)
-                hipLaunchKernelGGL(k_scale, dim3(nblk(V)), dim3(256), 0, c->stream, V, c->cr[cur], m0[1] / s_c);
-            if ((rc = nicr_field_stats(c, NICR_FIELD_O, &so)) || (rc = nicr_field_stats(c, NICR_FIELD_CR, &scr)))
-                return rc;
+            for (int k = 0; k < 2; ++k)
+                if (s1[k] > 0. && m0[k] > 0.) f[k] = m0[k] / s1[k];
+            hipLaunchKernelGGL(k_scale2, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], f[0], c->cr[cur], f[1]);
+            HIPCHK(hipStreamSynchronize(n->stream));
         }
+        const double omin = (double)pa[2] * f[0], omax = -(double)pa[3] * f[0];
+        const double cmin = (double)pa[4] * f[1], cmax = -(double)pa[5] * f[1];
+        const double pmin = (double)pa[6], pmax = -(double)pa[7];
         const double lo = opt->min_value, hi = opt->max_value;
-        if (!(so.min >= lo && so.max <= hi && scr.min >= lo && scr.max <= hi && sp.min >= lo && sp.max <= hi))
-            ok = false;
+        bool ok = std::isfinite(s1[0]) && std::isfinite(s1[1]) && omin >= lo && omax <= hi && cmin >= lo &&
+                  cmax <= hi && pmin >= lo && pmax <= hi;
         double ro = INFINITY, rcr = INFINITY;
         if (ok) {
             nicr_step_report rr{};
@@ -1438,9 +1558,12 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         }
         if (ok) {
             const int k = c->cur;   // proposal in k ^ 1, one step after it in k
-            if ((rc = sur_trap(n, c, c->o[k ^ 1], c->o[k], n->bk[0], n->du[0], J, &ro)) ||
-                (rc = sur_trap(n, c, c->cr[k ^ 1], c->cr[k], n->bk[1], n->du[1], J, &rcr)))
-                return rc;
+            hipLaunchKernelGGL(k_sur_trap, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, V, c->o[k ^ 1], c->o[k],
+                               n->bk[0], n->du[0], c->cr[k ^ 1], c->cr[k], n->bk[1], n->du[1], 0.5 * J, n->part);
+            long double pt[4];
+            if ((rc = sur_partials(n, 4, pt))) return rc;
+            ro = pt[1] > 0 ? (double)std::sqrt(pt[0] / pt[1]) : INFINITY;
+            rcr = pt[3] > 0 ? (double)std::sqrt(pt[2] / pt[3]) : INFINITY;
             if (!(ro <= opt->residual_tol && rcr <= opt->residual_tol)) ok = false;
         }
         if (ok) {

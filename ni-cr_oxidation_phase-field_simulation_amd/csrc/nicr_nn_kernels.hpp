@@ -159,33 +159,37 @@ __global__ __launch_bounds__(256) void k_conv(Geo g, const bf16* __restrict__ X,
 }
 
 // Halo-tile forward convolution for volumes whose sides are multiples of
-// (4, 8, 16). A block of 8 waves owns a run of units (output tile of 4 x 8 x 16
-// positions, 32 output channels); per stage (unit, 32-channel input chunk) it
-// holds in LDS the 6 x 10 x 18 halo of the tile and the chunk's weights for all
-// 27 taps, so every tap reads its A rows from LDS instead of HBM/L2. The next
-// stage's halo and weights are loaded into registers while the current one
-// computes. LDS rows are 64 bytes (32 channels) whose four 16-byte quads are
-// XOR-swizzled by (row >> 2) & 3, so 16 consecutive rows read by ds_read_b128
-// touch 16 distinct bank groups. Wave w computes the 2 x 32 positions at tile
-// x = w / 2, y = 4 (w % 2) .. + 3, z = 0 .. 15 against the unit's 32 channels.
-constexpr int HT_X = 4, HT_Y = 8, HT_Z = 16;
+// (4, 16, 16). A block of 8 waves owns a run of units (output tile of 4 x 16 x
+// 16 positions, 32 output channels); per stage (unit, 32-channel input chunk)
+// it holds the 6 x 18 x 18 halo of the tile in LDS, so every tap reads its A
+// rows from LDS instead of HBM/L2. The weights (B operand, 1 KB per
+// tap and K step, shared by every block) are read from L2 straight into
+// registers, two taps ahead, so LDS carries only A: each wave computes four
+// 32-position row-blocks per B fragment. LDS rows are 64 bytes (32 channels)
+// at a pitch of 80 bytes: 16-byte reads of consecutive rows fall in distinct
+// bank groups (5 row mod 16 and mod 8 are both permutations), and within each
+// of the 9 (x, y) tap groups every z tap's and row-block's address is the
+// group base plus a compile-time offset, so the reads are ds_read_b128 with
+// immediate offsets, the A fragments of the next step loading while the
+// current step's MFMAs run. Wave w computes 4 consecutive row-blocks (2 y rows
+// x 16 z each) of one x plane of the tile.
+constexpr int HT_X = 4, HT_Y = 16, HT_Z = 16;
 constexpr int HH_X = HT_X + 2, HH_Y = HT_Y + 2, HH_Z = HT_Z + 2;
-constexpr int H_ROWS = HH_X * HH_Y * HH_Z;      // 1080 halo positions
-constexpr int HW_ROWS = 27 * 32;                // weights: (tap, co) rows
-constexpr int HC_THREADS = 512;
-constexpr int H_LOADS = (H_ROWS * 4 + HC_THREADS - 1) / HC_THREADS;
-constexpr int HW_LOADS = (HW_ROWS * 4 + HC_THREADS - 1) / HC_THREADS;
-constexpr int HC_LDS_BYTES = (H_ROWS + HW_ROWS) * 64;
+constexpr int H_ROWS = HH_X * HH_Y * HH_Z;      // 1944 halo positions
+constexpr int H_PITCH = 80;                     // bytes per LDS row (64 used)
+constexpr int HC_LDS_BYTES = H_ROWS * H_PITCH;  // 155520
+constexpr int HC_RB = HT_X * HT_Y / 2;          // 32-position row-blocks per tile
+constexpr int HC_PF = 2;                        // weight prefetch distance (taps)
 
-__device__ __forceinline__ int hswz(int row, int q) { return row * 64 + ((q ^ ((row >> 2) & 3)) << 4); }
+constexpr int HC_THREADS = 512;                 // 8 waves
+constexpr int H_LOADS = (H_ROWS * 4 + HC_THREADS - 1) / HC_THREADS;
+constexpr int HC_MT = HC_RB / (HC_THREADS / 64);  // row-blocks per wave: 4
 
 template <bool RELU>
 __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __restrict__ X,
                                                           const bf16* __restrict__ W, const float* __restrict__ bias,
                                                           bf16* __restrict__ Y, int units, int upb) {
     extern __shared__ __align__(16) unsigned char lds[];
-    unsigned char* sH = lds;
-    unsigned char* sW = lds + H_ROWS * 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ntx = g.s0 / HT_X, nty = g.s1 / HT_Y, ntz = g.s2 / HT_Z;
     const int nco = g.cout / 32, nch = g.cin / 32;
@@ -193,11 +197,10 @@ __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __r
     if (u0 >= u1) return;
     const int nstage = (u1 - u0) * nch;
     const int plane = g.s1 * g.s2;
-    uint4 rh[H_LOADS], rw[HW_LOADS];
-    // global loads of stage s into registers
+    uint4 rh[H_LOADS];
+    // global loads of the halo of stage s into registers
     auto load = [&](int s) {
         const int u = u0 + s / nch, ch = s % nch;
-        const int cob = u % nco;
         int t = u / nco;
         const int tz = t % ntz;
         t /= ntz;
@@ -223,60 +226,70 @@ __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __r
             }
             rh[i] = v;
         }
-#pragma unroll
-        for (int i = 0; i < HW_LOADS; ++i) {
-            const int idx = tid + i * HC_THREADS;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (idx < HW_ROWS * 4) {
-                const int row = idx >> 2, q = idx & 3;
-                const int tap = row >> 5, co = row & 31;
-                v = *reinterpret_cast<const uint4*>(W + ((long long)tap * g.cout + cob * 32 + co) * g.cin + ch * 32 +
-                                                    q * 8);
-            }
-            rw[i] = v;
-        }
     };
     auto store = [&]() {
 #pragma unroll
         for (int i = 0; i < H_LOADS; ++i) {
             const int idx = tid + i * HC_THREADS;
-            if (idx < H_ROWS * 4) *reinterpret_cast<uint4*>(sH + hswz(idx >> 2, idx & 3)) = rh[i];
-        }
-#pragma unroll
-        for (int i = 0; i < HW_LOADS; ++i) {
-            const int idx = tid + i * HC_THREADS;
-            if (idx < HW_ROWS * 4) *reinterpret_cast<uint4*>(sW + hswz(idx >> 2, idx & 3)) = rw[i];
+            if (idx < H_ROWS * 4) *reinterpret_cast<uint4*>(lds + (idx >> 2) * H_PITCH + (idx & 3) * 16) = rh[i];
         }
     };
-    // this lane's rows: position r = lane % 32 of the wave's two row-blocks
-    const int wx = wave >> 1, wy = 4 * (wave & 1);
+    // this lane's rows: position r = lane % 32 of the wave's row-blocks; its
+    // LDS base covers row-block 0, tap (0, 0, 0), K quad hq
+    const int wx = (wave * HC_MT) / (HT_Y / 2), wy = 2 * ((wave * HC_MT) % (HT_Y / 2));
     const int r = lane & 31, hq = lane >> 5;
-    const int hrow0 = (wx * HH_Y + wy + (r >> 4)) * HH_Z + (r & 15);   // row-block 0, tap (0, 0, 0)
-    f32x16 acc0 = f32x16{}, acc1 = f32x16{};
+    const unsigned char* abase = lds + ((wx * HH_Y + wy + (r >> 4)) * HH_Z + (r & 15)) * H_PITCH + hq * 16;
+    const long long wtap = (long long)g.cout * g.cin;   // weight elements per tap
+    f32x16 acc[HC_MT];
+#pragma unroll
+    for (int m = 0; m < HC_MT; ++m) acc[m] = f32x16{};
     load(0);
     store();
     __syncthreads();
     for (int s = 0; s < nstage; ++s) {
         const bool more = s + 1 < nstage;
-        if (more) load(s + 1);
-#pragma unroll 3
-        for (int tap = 0; tap < 27; ++tap) {
-            const int a = tap / 9, b = (tap / 3) % 3, c = tap % 3;
-            const int hr = hrow0 + (a * HH_Y + b) * HH_Z + c;
-            const int wr = tap * 32 + r;
+        const int u = u0 + s / nch, ch = s % nch;
+        const int cob = u % nco;
+        // B fragment of (tap, k step kk) for this lane: 8 channels of k of output channel cob * 32 + r
+        const bf16* wl = W + (long long)(cob * 32 + r) * g.cin + ch * 32 + hq * 8;
+        uint4 bw[HC_PF + 1][2];
 #pragma unroll
-            for (int kk = 0; kk < 2; ++kk) {
-                const int q = kk * 2 + hq;
-                const bf16x8 bw = as_bf16x8(*reinterpret_cast<const uint4*>(sW + hswz(wr, q)));
-                const bf16x8 a0 = as_bf16x8(*reinterpret_cast<const uint4*>(sH + hswz(hr, q)));
-                const bf16x8 a1 = as_bf16x8(*reinterpret_cast<const uint4*>(sH + hswz(hr + 2 * HH_Z, q)));
-                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bw, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bw, acc1, 0, 0, 0);
+        for (int t = 0; t < HC_PF; ++t)
+#pragma unroll
+            for (int kk = 0; kk < 2; ++kk) bw[t][kk] = *reinterpret_cast<const uint4*>(wl + t * wtap + kk * 16);
+        // taps in 9 groups ab = 3a + b of 3 (c = 0, 1, 2); inside a group the A
+        // fragment of step (c, kk) and row-block m is at the group base plus a
+        // compile-time offset, and the weight ring slot of tap 3 ab + c is c
+        static_assert(HC_PF + 1 == 3, "weight ring of 3 slots: slot c for tap 3 ab + c");
+        auto aoff = [](int c, int kk, int m) { return (2 * m * HH_Z + c) * H_PITCH + kk * 32; };
+#pragma unroll 1
+        for (int ab = 0; ab < 9; ++ab) {
+            const unsigned char* gbase = abase + ((ab / 3) * HH_Y + ab % 3) * HH_Z * H_PITCH;
+            uint4 af[2][HC_MT];
+#pragma unroll
+            for (int m = 0; m < HC_MT; ++m) af[0][m] = *reinterpret_cast<const uint4*>(gbase + aoff(0, 0, m));
+#pragma unroll
+            for (int st = 0; st < 6; ++st) {
+                const int c = st >> 1, kk = st & 1;
+                const int tap = ab * 3 + c;
+                if (kk == 0 && tap + HC_PF < 27) {
+#pragma unroll
+                    for (int k2 = 0; k2 < 2; ++k2)
+                        bw[(c + HC_PF) % 3][k2] = *reinterpret_cast<const uint4*>(wl + (tap + HC_PF) * wtap + k2 * 16);
+                }
+                if (st + 1 < 6) {
+#pragma unroll
+                    for (int m = 0; m < HC_MT; ++m)
+                        af[(st + 1) & 1][m] =
+                            *reinterpret_cast<const uint4*>(gbase + aoff((st + 1) >> 1, (st + 1) & 1, m));
+                }
+                const bf16x8 bfr = as_bf16x8(bw[c][kk]);
+#pragma unroll
+                for (int m = 0; m < HC_MT; ++m)
+                    acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(af[st & 1][m]), bfr, acc[m], 0, 0, 0);
             }
         }
-        if (s % nch == nch - 1) {
-            const int u = u0 + s / nch;
-            const int cob = u % nco;
+        if (ch == nch - 1) {
             int t = u / nco;
             const int tz = t % ntz;
             t /= ntz;
@@ -287,21 +300,21 @@ __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __r
             const float bv = bias ? bias[co] : 0.f;
             const long long vbase = (long long)n * g.s0 * plane;
 #pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int pr = (e >> 2) * 8 + hq * 4 + (e & 3);   // row within the row-block
-                const int y = ty * HT_Y + wy + (pr >> 4), z = tz * HT_Z + (pr & 15), x = tx * HT_X + wx;
-                const long long p0 = vbase + (long long)x * plane + y * g.s2 + z;
-                float v0 = acc0[e] + bv, v1 = acc1[e] + bv;
-                if (RELU) {
-                    v0 = v0 > 0.f ? v0 : 0.f;
-                    v1 = v1 > 0.f ? v1 : 0.f;
+            for (int m = 0; m < HC_MT; ++m)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int pr = (e >> 2) * 8 + hq * 4 + (e & 3);   // row within the row-block
+                    const int y = ty * HT_Y + wy + 2 * m + (pr >> 4), z = tz * HT_Z + (pr & 15), x = tx * HT_X + wx;
+                    float v = acc[m][e] + bv;
+                    if (RELU) v = v > 0.f ? v : 0.f;
+                    Y[(vbase + (long long)x * plane + y * g.s2 + z) * g.cout + co] = (bf16)v;
                 }
-                Y[p0 * g.cout + co] = (bf16)v0;
-                Y[(p0 + 2 * g.s2) * g.cout + co] = (bf16)v1;
-            }
-            acc0 = f32x16{};
-            acc1 = f32x16{};
+#pragma unroll
+            for (int m = 0; m < HC_MT; ++m) acc[m] = f32x16{};
         }
+        // the next stage's halo: loaded after the compute, so its loads do not
+        // share the in-order vmcnt counter with the weight loads the MFMAs wait on
+        if (more) load(s + 1);
         __syncthreads();
         if (more) store();
         __syncthreads();

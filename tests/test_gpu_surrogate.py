@@ -52,11 +52,11 @@ CONV_CASES = [
     (1, 2, 4, 4, 384, 128),
     (2, 2, 2, 2, 256, 256),   # bottleneck of a 16^3 volume
     (1, 3, 1, 1, 32, 64),     # a single Y/Z cell: every wrapped tap is the cell itself
-    # sides multiples of (4, 8, 16): the halo-tile kernel
-    (1, 4, 8, 16, 32, 32),    # one tile: every halo row wraps in Y and Z, X zero padded
-    (2, 8, 8, 16, 96, 64),    # several input chunks, two output-channel units
+    # sides multiples of (4, 16, 16): the halo-tile kernel
+    (1, 4, 16, 16, 32, 32),   # one tile: every halo row wraps in Y and Z, X zero padded
+    (2, 8, 16, 16, 96, 64),   # several input chunks, two output-channel units
     (1, 4, 16, 32, 64, 32),
-    (1, 8, 8, 32, 384, 128),
+    (1, 8, 32, 16, 384, 128),
 ]
 
 
