@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void k_conv(Geo g, const bf16* __restrict__ X,
 // it holds the 6 x 18 x 18 halo of the tile in LDS, so every tap reads its A
 // rows from LDS instead of HBM/L2. The weights (B operand, 1 KB per
 // tap and K step, shared by every block) are read from L2 straight into
-// registers, two taps ahead, so LDS carries only A: each wave computes four
+// registers one 3-tap group ahead, so LDS carries only A: each wave computes four
 // 32-position row-blocks per B fragment. LDS rows are 64 bytes (32 channels)
 // at a pitch of 80 bytes: 16-byte reads of consecutive rows fall in distinct
 // bank groups (5 row mod 16 and mod 8 are both permutations), and within each
@@ -179,7 +179,6 @@ constexpr int H_ROWS = HH_X * HH_Y * HH_Z;      // 1944 halo positions
 constexpr int H_PITCH = 80;                     // bytes per LDS row (64 used)
 constexpr int HC_LDS_BYTES = H_ROWS * H_PITCH;  // 155520
 constexpr int HC_RB = HT_X * HT_Y / 2;          // 32-position row-blocks per tile
-constexpr int HC_PF = 2;                        // weight prefetch distance (taps)
 
 constexpr int HC_THREADS = 512;                 // 8 waves
 constexpr int H_LOADS = (H_ROWS * 4 + HC_THREADS - 1) / HC_THREADS;
@@ -252,18 +251,25 @@ __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __r
         const int cob = u % nco;
         // B fragment of (tap, k step kk) for this lane: 8 channels of k of output channel cob * 32 + r
         const bf16* wl = W + (long long)(cob * 32 + r) * g.cin + ch * 32 + hq * 8;
-        uint4 bw[HC_PF + 1][2];
+        // weights of the 3 taps of a group (c = 0, 1, 2; K steps 0, 1), loaded
+        // one group ahead: the next group's fragments are in flight for the 24
+        // MFMAs of the current group
+        auto load_group = [&](int ab, uint4 (&w)[3][2]) {
 #pragma unroll
-        for (int t = 0; t < HC_PF; ++t)
+            for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int kk = 0; kk < 2; ++kk) bw[t][kk] = *reinterpret_cast<const uint4*>(wl + t * wtap + kk * 16);
+                for (int kk = 0; kk < 2; ++kk)
+                    w[c][kk] = *reinterpret_cast<const uint4*>(wl + (ab * 3 + c) * wtap + kk * 16);
+        };
+        uint4 bw[3][2], bn[3][2];
+        load_group(0, bw);
         // taps in 9 groups ab = 3a + b of 3 (c = 0, 1, 2); inside a group the A
         // fragment of step (c, kk) and row-block m is at the group base plus a
-        // compile-time offset, and the weight ring slot of tap 3 ab + c is c
-        static_assert(HC_PF + 1 == 3, "weight ring of 3 slots: slot c for tap 3 ab + c");
+        // compile-time offset
         auto aoff = [](int c, int kk, int m) { return (2 * m * HH_Z + c) * H_PITCH + kk * 32; };
 #pragma unroll 1
         for (int ab = 0; ab < 9; ++ab) {
+            if (ab + 1 < 9) load_group(ab + 1, bn);
             const unsigned char* gbase = abase + ((ab / 3) * HH_Y + ab % 3) * HH_Z * H_PITCH;
             uint4 af[2][HC_MT];
 #pragma unroll
@@ -271,12 +277,6 @@ __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __r
 #pragma unroll
             for (int st = 0; st < 6; ++st) {
                 const int c = st >> 1, kk = st & 1;
-                const int tap = ab * 3 + c;
-                if (kk == 0 && tap + HC_PF < 27) {
-#pragma unroll
-                    for (int k2 = 0; k2 < 2; ++k2)
-                        bw[(c + HC_PF) % 3][k2] = *reinterpret_cast<const uint4*>(wl + (tap + HC_PF) * wtap + k2 * 16);
-                }
                 if (st + 1 < 6) {
 #pragma unroll
                     for (int m = 0; m < HC_MT; ++m)
@@ -288,6 +288,10 @@ __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __r
                 for (int m = 0; m < HC_MT; ++m)
                     acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(af[st & 1][m]), bfr, acc[m], 0, 0, 0);
             }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int kk = 0; kk < 2; ++kk) bw[c][kk] = bn[c][kk];
         }
         if (ch == nch - 1) {
             int t = u / nco;
