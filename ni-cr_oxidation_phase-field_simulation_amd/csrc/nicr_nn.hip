@@ -608,18 +608,22 @@ static bool halo_ready() {
     return g_halo.enabled;
 }
 
-void launch_conv(Net* n, const Geo& g, const bf16* X, const bf16* W, const float* bias, bf16* Y, bool relu) {
-    if (g.s0 % HT_X == 0 && g.s1 % HT_Y == 0 && g.s2 % HT_Z == 0 && halo_ready()) {
+bool halo_applies(const Geo& g) { return g.s0 % HT_X == 0 && g.s1 % HT_Y == 0 && g.s2 % HT_Z == 0 && halo_ready(); }
+
+// Xc: coarse source of channels 0 .. g.cu - 1 (halo kernel only)
+void launch_conv(Net* n, const Geo& g, const bf16* X, const bf16* W, const float* bias, bf16* Y, bool relu,
+                 const bf16* Xc = nullptr) {
+    if (halo_applies(g)) {
         const int units = g.nb * (g.s0 / HT_X) * (g.s1 / HT_Y) * (g.s2 / HT_Z) * (g.cout / 32);
         const int nblock = std::min(units, g_halo.ncu);
         const int upb = (units + nblock - 1) / nblock;
         const int grid = (units + upb - 1) / upb;
         if (relu)
             hipLaunchKernelGGL(k_conv_halo<true>, dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W, bias,
-                               Y, units, upb);
+                               Y, units, upb, Xc);
         else
             hipLaunchKernelGGL(k_conv_halo<false>, dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W,
-                               bias, Y, units, upb);
+                               bias, Y, units, upb, Xc);
         return;
     }
     const int P = g.positions();
@@ -783,6 +787,19 @@ void forward(Net* n, int nb) {
         hipLaunchKernelGGL(k_up_cat, dim3(nblk(np * ((Cu + Cs) / 8))), dim3(256), 0, s, (int)np, n->S[0] >> level,
                            n->S[1] >> level, n->S[2] >> level, Cu, Cs, L, Sk, U);
     };
+    // decoder convolution i on [up(L), Sk]: without training buffers and where
+    // the halo kernel applies, the concatenation is read in place (never stored)
+    auto upcat_conv = [&](int i, int Cu, const bf16* L, const bf16* Sk) {
+        const ConvL& Lc = c[i];
+        Geo g = geo(n, nb, Lc.level, Lc.cin, Lc.cout, 27);
+        if (!n->train && halo_applies(g)) {
+            g.cu = Cu;
+            launch_conv(n, g, Sk, Lc.Wb, n->param + Lc.b_off, Lc.y, Lc.relu, L);
+        } else {
+            upcat(Lc.level, Cu, Lc.cin - Cu, L, Sk, Lc.x);
+            conv(i);
+        }
+    };
     if (n->timing) hipEventRecord(n->ev[0], s);
     conv(0);
     conv(1);
@@ -795,14 +812,11 @@ void forward(Net* n, int nb) {
     pool(2, 128, n->e3, n->p3);
     conv(6);
     conv(7);
-    upcat(2, 256, 128, n->b4, n->e3, n->u3);
-    conv(8);
+    upcat_conv(8, 256, n->b4, n->e3);
     conv(9);
-    upcat(1, 128, 64, n->d3, n->e2, n->u2);
-    conv(10);
+    upcat_conv(10, 128, n->d3, n->e2);
     conv(11);
-    upcat(0, 64, 32, n->d2, n->e1, n->u1);
-    conv(12);
+    upcat_conv(12, 64, n->d2, n->e1);
     conv(13);
     const long long P0 = (long long)nb * n->V(0);
     hipLaunchKernelGGL(k_head, dim3(nblk(P0)), dim3(256), 0, s, (int)P0, n->d1, n->param + n->head_w,

@@ -31,6 +31,10 @@ struct Geo {
     int nb, s0, s1, s2;   // volumes and their cells (s2 fastest)
     int cin, cout;        // multiples of 32
     int taps;             // 27 (3x3x3) or 1
+    // k_conv_halo only: input channels 0 .. cu-1 come from a coarse tensor of cu
+    // channels (nearest upsampling: the cell's parent at half the sides), the
+    // rest from X with cin - cu channels (0: X alone)
+    int cu = 0;
     __host__ __device__ int positions() const { return nb * s0 * s1 * s2; }
 };
 
@@ -187,7 +191,8 @@ constexpr int HC_MT = HC_RB / (HC_THREADS / 64);  // row-blocks per wave: 4
 template <bool RELU>
 __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __restrict__ X,
                                                           const bf16* __restrict__ W, const float* __restrict__ bias,
-                                                          bf16* __restrict__ Y, int units, int upb) {
+                                                          bf16* __restrict__ Y, int units, int upb,
+                                                          const bf16* __restrict__ Xc) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ntx = g.s0 / HT_X, nty = g.s1 / HT_Y, ntz = g.s2 / HT_Z;
@@ -207,7 +212,12 @@ __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __r
         t /= nty;
         const int tx = t % ntx, n = t / ntx;
         const int x0 = tx * HT_X - 1, y0 = ty * HT_Y - 1, z0 = tz * HT_Z - 1;
-        const long long vbase = (long long)n * g.s0 * plane;
+        // the chunk's source: the coarse tensor (upsampled) or X
+        const bool coarse = ch * 32 < g.cu;
+        const int h1 = g.s1 >> 1, h2 = g.s2 >> 1;
+        const long long vbase = coarse ? (long long)n * (g.s0 >> 1) * h1 * h2 : (long long)n * g.s0 * plane;
+        const bf16* src = coarse ? Xc + ch * 32 : X + (ch * 32 - g.cu);
+        const int stride = coarse ? g.cu : g.cin - g.cu;
 #pragma unroll
         for (int i = 0; i < H_LOADS; ++i) {
             const int idx = tid + i * HC_THREADS;
@@ -219,9 +229,11 @@ __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __r
                 int gy = y0 + hy, gz = z0 + hz;
                 gy = gy < 0 ? gy + g.s1 : (gy >= g.s1 ? gy - g.s1 : gy);
                 gz = gz < 0 ? gz + g.s2 : (gz >= g.s2 ? gz - g.s2 : gz);
-                if (gx >= 0 && gx < g.s0)
-                    v = *reinterpret_cast<const uint4*>(
-                        X + (vbase + (long long)gx * plane + gy * g.s2 + gz) * g.cin + ch * 32 + q * 8);
+                if (gx >= 0 && gx < g.s0) {
+                    const long long pos = coarse ? vbase + ((long long)(gx >> 1) * h1 + (gy >> 1)) * h2 + (gz >> 1)
+                                                 : vbase + (long long)gx * plane + gy * g.s2 + gz;
+                    v = *reinterpret_cast<const uint4*>(src + pos * stride + q * 8);
+                }
             }
             rh[i] = v;
         }

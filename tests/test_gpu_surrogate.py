@@ -160,6 +160,33 @@ def test_unet_forward_and_gradients_vs_torch(shape, nb):
     net.close()
 
 
+def test_inference_network_equals_training_network():
+    """Without training buffers the decoder convolutions read the skip
+    concatenation in place (coarse tensor upsampled + skip tensor) instead of
+    a stored concatenation: the outputs must be bitwise those of the network
+    that stores it. 16 x 32 x 32 has halo-kernel levels 0 and 1 and a per-tap
+    level 2."""
+    sur = _sur()
+    shape = (16, 32, 32)
+    a = sur.UNet(shape, max_batch=1, train=True, seed=7)
+    b = sur.UNet(shape, max_batch=1, train=False, seed=7)
+    p = _random_net_params(a, 5)
+    a.set_params(p)
+    b.set_params(p)
+    norm = dict(phi_mean=0.5, phi_scale=0.3, gb_mean=0.5, gb_scale=0.3, c_mean=[-3.5, 0.2, -7.0, -6.0],
+                c_scale=[0.5, 0.02, 0.5, 0.5], gain=[1.0, 1.0, 1.0], jump=50)
+    a.set_norm(**norm)
+    b.set_norm(**norm)
+    rng = np.random.default_rng(9)
+    sz = (1,) + shape
+    fields = np.stack([rng.uniform(0, 2e-3, size=sz), 0.2 + 0.01 * rng.normal(size=sz), rng.uniform(0, 1, size=sz),
+                       rng.uniform(0, 1, size=sz), 1e-7 * rng.normal(size=sz), 1e-6 * rng.normal(size=sz),
+                       1e-4 * rng.normal(size=sz)], axis=1)
+    assert np.array_equal(a.forward(fields), b.forward(fields))
+    a.close()
+    b.close()
+
+
 def _closed_model(nicr, shape, seed, **kw):
     o, cr, p, gb = _syn().closed_alloy(shape, n_seeds=8, seed=seed)
     cfg = nicr.make_config(3, nx=shape[0], ny=shape[1], nz=shape[2], grains=True, grain_evolution=False, bc="closed",
