@@ -295,7 +295,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-surrogate measurement")
     ap.add_argument("--surrogate-only", action="store_true", help="only the learned-surrogate measurement")
     ap.add_argument("--sur-traj", type=int, default=200, help="training trajectories of the surrogate")
-    ap.add_argument("--sur-iters", type=int, default=2000, help="Adam iterations of the surrogate")
+    ap.add_argument("--sur-iters", type=int, default=3000, help="Adam iterations of the surrogate")
     ap.add_argument("--sur-train-n", type=int, default=128, help="side of the training volumes")
     ap.add_argument("--sur-run-n", type=int, default=256, help="side of the surrogate run")
     ap.add_argument("--sur-cycles", type=int, default=20, help="jump cycles (52 steps each) of the timed run")

@@ -340,10 +340,24 @@ __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __r
 // Weight gradient: dW[tap][co][ci] += sum_p dY[p][co] X[p + off(tap)][ci] in FP32
 // (atomics). Block: one tap, a 32 x 32 (co, ci) tile and a chunk of positions;
 // its 4 waves take interleaved 32-position stages of the chunk. Per stage a wave
-// stages 32 rows of dY and of X in its own LDS slice; the A operand (co x
-// positions) and the B operand (positions x ci) are read column-wise from there.
+// stages 32 rows of dY and of X ([position][channel], channels-last as in HBM)
+// in its own LDS slice. The MFMA operands need 8 consecutive positions of one
+// channel per lane (K = positions): ds_read_b64_tr_b16 delivers them from the
+// channels-last rows. Within a group of 16 lanes, lane i receives element i % 4
+// of the 8-byte chunks addressed by lanes 4 j + i / 4 (j = 0 .. 3), so lane m
+// of the group addresses row p0 + m / 4, channels cb + 4 (m % 4) .. + 3, and
+// lane i then holds channel cb + i at positions p0 .. p0 + 3 (mapping verified
+// on gfx950). Two such reads give the 8 K values of an operand.
 constexpr int WPC = 32;    // positions per wave stage
-constexpr int WLD = 40;    // LDS row pitch (bf16)
+constexpr int WLD = 40;    // LDS row pitch (bf16): rows stay 8-byte aligned
+
+typedef short v4i16 __attribute__((ext_vector_type(4)));
+
+// p points into LDS (a __shared__ array)
+__device__ __forceinline__ v4i16 lds_tr16(const bf16* p) {
+    return __builtin_amdgcn_ds_read_tr16_b64_v4i16((__attribute__((address_space(3))) v4i16*)(p));
+}
+
 __global__ __launch_bounds__(256) void k_conv_wgrad(Geo g, const bf16* __restrict__ X, const bf16* __restrict__ dY,
                                                     float* __restrict__ dW, int chunk) {
     __shared__ __align__(16) bf16 sY[4][WPC * WLD];
@@ -357,10 +371,30 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(Geo g, const bf16* __restric
     const int tap = b % g.taps;
     const int c0 = (b / g.taps) * chunk;
     const int co0 = (tile / nci) * 32, ci0 = (tile % nci) * 32;
+    const int d0 = g.taps == 1 ? 0 : tap / 9 - 1, d1 = g.taps == 1 ? 0 : (tap / 3) % 3 - 1,
+              d2 = g.taps == 1 ? 0 : tap % 3 - 1;
     f32x16 acc = f32x16{};
     // lane loads: position lane >> 1, 16 channels at (lane & 1) * 16
     const int lp = lane >> 1, lh = lane & 1;
+    // transposed operand reads: group gq = lane / 16 -> channel base 16 (gq & 1),
+    // positions (gq >> 1) * 8 of the K step; lane m of the group addresses row
+    // m / 4, channels 4 (m % 4)
+    const int gq = lane >> 4, m = lane & 15;
+    const int toff = ((gq >> 1) * 8 + (m >> 2)) * WLD + 16 * (gq & 1) + 4 * (m & 3);
     const int cend = min(c0 + chunk, P);
+    // coordinates of this lane's position, advanced by the stage stride (no
+    // division inside the loop)
+    int pn, i0, i1, i2;
+    {
+        const int p = min(c0 + wave * WPC + lp, P - 1);
+        const int vol = g.s0 * g.s1 * g.s2, plane = g.s1 * g.s2;
+        pn = p / vol;
+        const int rem = p - pn * vol;
+        i0 = rem / plane;
+        const int rem2 = rem - i0 * plane;
+        i1 = rem2 / g.s2;
+        i2 = rem2 - i1 * g.s2;
+    }
     for (int p0 = c0 + wave * WPC; p0 < cend; p0 += 4 * WPC) {
         const int p = p0 + lp;
         uint4 y0 = make_uint4(0u, 0u, 0u, 0u), y1 = y0, x0 = y0, x1 = y0;
@@ -368,9 +402,11 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(Geo g, const bf16* __restric
             const uint4* sy = reinterpret_cast<const uint4*>(dY + (long long)p * g.cout + co0 + 16 * lh);
             y0 = sy[0];
             y1 = sy[1];
-            Nbr nb;
-            nb.init(g, p);
-            const int q = nb.at(g.taps, tap);
+            const int j0 = i0 + d0;
+            int j1 = i1 + d1, j2 = i2 + d2;
+            j1 = j1 < 0 ? j1 + g.s1 : (j1 >= g.s1 ? j1 - g.s1 : j1);
+            j2 = j2 < 0 ? j2 + g.s2 : (j2 >= g.s2 ? j2 - g.s2 : j2);
+            const int q = (j0 >= 0 && j0 < g.s0) ? ((pn * g.s0 + j0) * g.s1 + j1) * g.s2 + j2 : -1;
             if (q >= 0) {
                 const uint4* sx = reinterpret_cast<const uint4*>(X + (long long)q * g.cin + ci0 + 16 * lh);
                 x0 = sx[0];
@@ -388,18 +424,28 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(Geo g, const bf16* __restric
         asm volatile("" ::: "memory");
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
-            bf16x8 a, bb;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int pk = kk * 16 + (lane >> 5) * 8 + e;   // position within the stage (k)
-                a[e] = sY[wave][pk * WLD + (lane & 31)];          // A[m = co][k]
-                bb[e] = sX[wave][pk * WLD + (lane & 31)];         // B[k][n = ci]
-            }
+            const int o = kk * 16 * WLD + toff;
+            const v4i16 a0 = lds_tr16(&sY[wave][o]), a1 = lds_tr16(&sY[wave][o + 4 * WLD]);
+            const v4i16 b0 = lds_tr16(&sX[wave][o]), b1 = lds_tr16(&sX[wave][o + 4 * WLD]);
+            const bf16x8 a = __builtin_bit_cast(bf16x8, __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7));
+            const bf16x8 bb = __builtin_bit_cast(bf16x8, __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7));
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc, 0, 0, 0);
         }
         asm volatile("" ::: "memory");
         __builtin_amdgcn_s_waitcnt(0xc07f);   // reads done before the next stage overwrites the slice
         __builtin_amdgcn_wave_barrier();
+        // advance the coordinates by 4 * WPC positions
+        i2 += 4 * WPC;
+        while (i2 >= g.s2) {
+            i2 -= g.s2;
+            if (++i1 == g.s1) {
+                i1 = 0;
+                if (++i0 == g.s0) {
+                    i0 = 0;
+                    ++pn;
+                }
+            }
+        }
     }
     const int ci = ci0 + (lane & 31);
 #pragma unroll
