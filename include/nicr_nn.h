@@ -90,7 +90,9 @@ int nicr_sur_samples(void* net, int* count);
  * that give the normalised corrections unit mean square, and the jump. */
 int nicr_sur_fit_norm(void* net);
 /* `iters` Adam iterations (learning rate lr) on random batches of `batch`
- * samples drawn with `seed`; losses[iters] (mean squared error of the
+ * samples drawn with `seed`; the gradient is clipped to 3 times the running
+ * mean of its global norm, and an iteration with a non-finite gradient leaves
+ * the parameters unchanged. losses[iters] (mean squared error of the
  * normalised changes per iteration) when not NULL. */
 int nicr_sur_train(void* net, int iters, int batch, float lr, uint64_t seed, float* losses);
 /* Mean squared error of the normalised changes over samples [first, first+count). */

@@ -463,12 +463,43 @@ __global__ void k_delta_float(long long n, const double* __restrict__ a, const d
     if (i < n) out[i] = (float)(a[i] - b[i]);
 }
 
-// Adam (bias-corrected) on the FP32 master parameters.
+// Global gradient norm: acc[0] += sum g^2.
+__global__ __launch_bounds__(256) void k_grad_sq(long long n, const float* __restrict__ g, double* __restrict__ acc) {
+    __shared__ double red[4];
+    double s = 0.;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        s += (double)g[i] * (double)g[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(acc, red[0] + red[1] + red[2] + red[3]);
+}
+
+// Gradient clipping factor from the norm (clip[0] = ||g||^2 of this iteration):
+// scale = min(1, 3 ema / ||g||), with ema a running mean of the clipped norms
+// (clip[1]; 0 before the first iteration). clip[2] receives the scale.
+__global__ void k_clip_factor(double* clip) {
+    const double nrm = sqrt(clip[0]);
+    double ema = clip[1];
+    if (!(ema > 0.)) ema = nrm;
+    const double lim = 3. * ema;
+    double f = nrm > lim && nrm > 0. ? lim / nrm : 1.;
+    if (!(nrm == nrm) || nrm > 1e300) f = 0.;   // a non-finite gradient leaves the parameters unchanged
+    clip[2] = f;
+    if (f > 0.) clip[1] = 0.99 * ema + 0.01 * (nrm * f);
+    clip[0] = 0.;
+}
+
+// Adam (bias-corrected) on the FP32 master parameters, gradient scaled by *gscale.
 __global__ void k_adam(long long n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, float lr, float b1, float b2, float c1, float c2, float eps) {
+                       float* __restrict__ v, float lr, float b1, float b2, float c1, float c2, float eps,
+                       const double* __restrict__ gscale) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float gi = g[i];
+    const float sc = (float)*gscale;
+    if (sc == 0.f) return;
+    const float gi = g[i] * sc;
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
     m[i] = mi;
@@ -523,6 +554,7 @@ struct Net {
     float* loss_acc = nullptr;   // one slot per training iteration
     int loss_cap = 0;
     double* dacc = nullptr;      // reduction scratch
+    double* clip = nullptr;      // gradient clipping: norm^2 of the iteration, running mean, factor
     nicr_unet_norm norm{};
     std::vector<double> ds_stat;   // per training sample: mean and std of O and Cr
     // training set: inputs [S][4][V] and 50-step deltas [S][3][V] (FP32)
@@ -684,7 +716,14 @@ int net_build(Net* n) {
     if ((rc = nalloc_t(n, &n->param, n->nparam)) || (rc = nalloc_t(n, &n->grad, n->nparam)) ||
         (rc = nalloc_t(n, &n->dacc, 64)))
         return rc;
-    if (n->train && ((rc = nalloc_t(n, &n->am, n->nparam)) || (rc = nalloc_t(n, &n->av, n->nparam)))) return rc;
+    if (n->train && ((rc = nalloc_t(n, &n->am, n->nparam)) || (rc = nalloc_t(n, &n->av, n->nparam)) ||
+                     (rc = nalloc_t(n, &n->clip, 4))))
+        return rc;
+    if (n->train) {
+        HIPCHK(hipMemsetAsync(n->am, 0, n->nparam * 4, n->stream));
+        HIPCHK(hipMemsetAsync(n->av, 0, n->nparam * 4, n->stream));
+        HIPCHK(hipMemsetAsync(n->clip, 0, 4 * 8, n->stream));
+    }
     for (int i = 0; i < N_CONV; ++i) {
         ConvL& L = n->conv[i];
         const long long nw = 27LL * L.cin * L.cout;
@@ -882,13 +921,17 @@ void backward(Net* n, int nb, float* loss_slot) {
     conv_bwd(0, false, false);
 }
 
+// One Adam update with global-norm clipping (no host synchronisation: the
+// norm, the clip factor and its running mean stay on the device).
 void adam_step(Net* n, float lr) {
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     ++n->adam_t;
     const float c1 = 1.f / (1.f - std::pow(b1, (float)n->adam_t));
     const float c2 = 1.f / (1.f - std::pow(b2, (float)n->adam_t));
+    hipLaunchKernelGGL(k_grad_sq, dim3(512), dim3(256), 0, n->stream, n->nparam, n->grad, n->clip);
+    hipLaunchKernelGGL(k_clip_factor, dim3(1), dim3(1), 0, n->stream, n->clip);
     hipLaunchKernelGGL(k_adam, dim3(nblk(n->nparam)), dim3(256), 0, n->stream, n->nparam, n->param, n->grad, n->am,
-                       n->av, lr, b1, b2, c1, c2, eps);
+                       n->av, lr, b1, b2, c1, c2, eps, (const double*)(n->clip + 2));
     cast_weights(n);
 }
 
@@ -1064,6 +1107,7 @@ int nicr_unet_set_params(void* h, const float* host) {
     if (n->train) {
         HIPCHK(hipMemsetAsync(n->am, 0, n->nparam * 4, n->stream));
         HIPCHK(hipMemsetAsync(n->av, 0, n->nparam * 4, n->stream));
+        HIPCHK(hipMemsetAsync(n->clip, 0, 4 * 8, n->stream));
         HIPCHK(hipStreamSynchronize(n->stream));
     }
     return NICR_OK;
