@@ -194,12 +194,16 @@ def unet_flops_per_cell():
     return sum(2 * 27 * ci * co / 8 ** lv for lv, convs in levels.items() for ci, co in convs)
 
 
-def surrogate_bench(nicr, n_train, n_run, n_traj, iters, cycles, tol, seed):
-    """Learned surrogate on the C2-like closed workload: training set from
-    n_traj plain binary64 trajectories at n_train^3, Adam on the GPU, then
-    n_run^3 from one state (after 200 plain steps) advanced cycles * 52 steps by
-    the plain binary64 step and by surrogate jumps of 50 steps; both timed, the
-    surrogate's final fields and O/Cr totals compared with the plain run's."""
+def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, warm, tol, seed, n_eta, n_seeds, settle):
+    """Learned surrogate on the closed workload: training set from n_traj
+    plain binary64 trajectories at n_train^3, Adam on the GPU, then n_run^3 from
+    one state (after `warm` plain steps) advanced nsteps by the plain binary64
+    step and by surrogate jumps of 50 steps (in logged chunks of 520 steps);
+    both timed, the surrogate's final fields and O/Cr totals compared with the
+    plain run's.
+    n_eta > 0: n_seeds grains at n_run^3 coloured onto n_eta evolving eta fields
+    (training volumes hold the same grain size); n_eta = 0: static grains.
+    settle: plain steps after an accepted jump before the next proposal."""
     sur = importlib.import_module(PKG_NAME + ".surrogate")
     syn = importlib.import_module(PKG_NAME + ".synthetic")
 
@@ -207,8 +211,9 @@ def surrogate_bench(nicr, n_train, n_run, n_traj, iters, cycles, tol, seed):
         print(f"[bench] surrogate: {msg}", file=sys.stderr, flush=True)
 
     J = 50
+    train_seeds = max(2, round(n_seeds * (n_train / n_run) ** 3))
     net = sur.UNet((n_train,) * 3, max_batch=2, train=True, max_samples=n_traj, seed=seed)
-    rec = sur.record_trajectories(net, nicr, n_traj, jump=J, seed=seed, log=log)
+    rec = sur.record_trajectories(net, nicr, n_traj, jump=J, n_seeds=train_seeds, seed=seed, log=log, n_eta=n_eta)
     net.fit_norm()
     tr = sur.train_schedule(net, iters, batch=2, lr=1e-3, seed=seed, log=log)
     params, norm = net.params(), net.norm()
@@ -216,52 +221,82 @@ def surrogate_bench(nicr, n_train, n_run, n_traj, iters, cycles, tol, seed):
     shape = (n_run,) * 3
     cells = n_run ** 3
     t0 = time.perf_counter()
-    state = syn.closed_alloy(shape, n_seeds=16 * (n_run // n_train) ** 3, seed=seed + 7)
+    state = syn.closed_alloy(shape, n_seeds=n_seeds, seed=seed + 7, n_eta=n_eta)
     log(f"{n_run}^3 state built in {time.perf_counter() - t0:.1f} s")
     plain = sur.closed_model(nicr, shape, state)
     sm = sur.closed_model(nicr, shape, state)
     del state
     for m in (plain, sm):
-        r = m.step(200)
-        if r.steps_done != 200:
+        r = m.step(warm)
+        if r.steps_done != warm:
             raise RuntimeError("surrogate workload: warm-up step rejected")
     inet = sur.UNet(shape, seed=1)
     inet.set_params(params)
     inet.set_norm(norm)
     inet.set_timing(True)
     mass0 = [sum(sm.stats(f)[0]) for f in ("o", "cr")]
-    nsteps = cycles * (J + 2)
     plain.sync()
     t0 = time.perf_counter()
     r = plain.step(nsteps)
     plain.sync()
     t_plain = time.perf_counter() - t0
-    sm.sync()
-    t0 = time.perf_counter()
-    rep = inet.advance(sm, nsteps, jump=J, residual_tol=tol)
-    sm.sync()
-    t_sur = time.perf_counter() - t0
-    if r.steps_done != nsteps or rep.steps_done != nsteps or rep.status != 0:
+    log(f"plain binary64: {nsteps} steps in {t_plain:.2f} s")
+    tot = {"steps": 0, "acc": 0, "rej": 0, "plain": 0, "res": [0.0, 0.0, 0.0], "min_rej": None}
+    chunk = 10 * (J + 2)
+    t_sur = 0.0
+    while tot["steps"] < nsteps:
+        k = min(chunk, nsteps - tot["steps"])
+        sm.sync()
+        t0 = time.perf_counter()
+        rep = inet.advance(sm, k, jump=J, residual_tol=tol, settle_steps=settle)
+        sm.sync()
+        dt_chunk = time.perf_counter() - t0
+        t_sur += dt_chunk
+        if rep.steps_done != k or rep.status != 0:
+            raise RuntimeError("surrogate workload: a plain step was rejected")
+        tot["steps"] += k
+        tot["acc"] += rep.jumps_accepted
+        tot["rej"] += rep.jumps_rejected
+        tot["plain"] += rep.plain_steps
+        tot["res"] = [max(a, b) for a, b in zip(tot["res"], (rep.max_residual_o, rep.max_residual_cr,
+                                                             rep.max_residual_eta))]
+        if rep.jumps_rejected:
+            mr = rep.min_rejected_residual
+            tot["min_rej"] = mr if tot["min_rej"] is None else min(tot["min_rej"], mr)
+        log(f"steps {tot['steps'] - k}-{tot['steps']}: {rep.jumps_accepted} jumps accepted, {rep.jumps_rejected} "
+            f"rejected (smallest rejected residual {rep.min_rejected_residual:.3f}), {dt_chunk * 1e3:.1f} ms")
+    if r.steps_done != nsteps:
         raise RuntimeError("surrogate workload: a plain step was rejected")
     err = {}
     for f in ("o", "cr", "p"):
         a, b = sm.get(f), plain.get(f)
         nb = np.linalg.norm(b)
         err[f] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.abs(a - b).max())
+    if n_eta:
+        num = den = 0.0
+        for g in range(n_eta):
+            a, b = sm.get(g), plain.get(g)
+            num += float(np.sum((a - b) ** 2))
+            den += float(np.sum(b * b))
+        err["eta"] = (num / den) ** 0.5
     mass_sur = [sum(sm.stats(f)[0]) for f in ("o", "cr")]
     mass_plain = [sum(plain.stats(f)[0]) for f in ("o", "cr")]
     fwd_ms = inet.forward_ms()
     flops = unet_flops_per_cell() * cells
     dt = plain.c.dt
     out = {
-        "workload": f"3D {n_run}^3 closed, {16 * (n_run // n_train) ** 3} static Voronoi grains (GB mask), "
-                    f"Cr 0.20+-0.005, O 1e-3 (+-20%), no oxide; binary64 steps; jump {J} steps",
+        "workload": f"3D {n_run}^3 closed, " + (f"{n_eta} evolving eta from {n_seeds} Voronoi seeds"
+                                                  if n_eta else f"{n_seeds} static Voronoi grains (GB mask)")
+                    + f", Cr 0.20+-0.005, O 1e-3 (+-20%), no oxide; binary64 steps; jump {J} steps",
         "speedup": round(t_plain / t_sur, 3),
         "sim_time_per_s_plain_fp64": nsteps * dt / t_plain,
         "sim_time_per_s_surrogate": nsteps * dt / t_sur,
-        "steps": nsteps, "ms_plain": round(t_plain * 1e3, 2), "ms_surrogate": round(t_sur * 1e3, 2),
-        "jumps_accepted": rep.jumps_accepted, "jumps_rejected": rep.jumps_rejected, "plain_steps": rep.plain_steps,
-        "residual_tol": tol, "max_residual": [round(rep.max_residual_o, 4), round(rep.max_residual_cr, 4)],
+        "steps": nsteps, "warm_steps": warm, "ms_plain": round(t_plain * 1e3, 2),
+        "ms_surrogate": round(t_sur * 1e3, 2),
+        "jumps_accepted": tot["acc"], "jumps_rejected": tot["rej"], "plain_steps": tot["plain"],
+        "residual_tol": tol, "settle_steps": settle, "max_residual": [round(tot["res"][0], 4), round(tot["res"][1], 4)]
+                                             + ([float(f"{tot['res'][2]:.3e}")] if n_eta else []),
+        "min_rejected_residual": None if tot["min_rej"] is None else round(tot["min_rej"], 4),
         "rel_l2_vs_plain": {k: float(f"{v:.3e}") for k, v in err.items()},
         "mass_rel_vs_plain": [float(f"{abs(a - b) / abs(b):.3e}") for a, b in zip(mass_sur, mass_plain)],
         "mass_rel_vs_start": [float(f"{abs(a - b) / abs(b):.3e}") for a, b in zip(mass_sur, mass0)],
@@ -298,8 +333,13 @@ def main():
     ap.add_argument("--sur-iters", type=int, default=3000, help="Adam iterations of the surrogate")
     ap.add_argument("--sur-train-n", type=int, default=128, help="side of the training volumes")
     ap.add_argument("--sur-run-n", type=int, default=256, help="side of the surrogate run")
-    ap.add_argument("--sur-cycles", type=int, default=20, help="jump cycles (52 steps each) of the timed run")
+    ap.add_argument("--sur-steps", type=int, default=5000, help="steps of the timed run (5000: the 256^3 run length)")
+    ap.add_argument("--sur-warm", type=int, default=0, help="plain steps before the timed run")
+    ap.add_argument("--sur-settle", type=int, default=5,
+                    help="plain steps after an accepted jump before the next proposal")
     ap.add_argument("--sur-tol", type=float, default=0.1, help="trapezoid residual tolerance of a jump")
+    ap.add_argument("--sur-static", action="store_true",
+                    help="surrogate on static grains (128 seeds at 256^3) instead of the --eta evolving fields")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the 1 GiB device copy reference (rocprof counter calibration)")
     args = ap.parse_args()
@@ -316,7 +356,9 @@ def main():
         return
     def run_surrogate():
         return surrogate_bench(nicr, args.sur_train_n, args.sur_run_n, args.sur_traj, args.sur_iters,
-                               args.sur_cycles, args.sur_tol, 20260909)
+                               args.sur_steps, args.sur_warm, args.sur_tol, 20260909, 0 if args.sur_static else args.eta,
+                               128 * (args.sur_run_n // 256) ** 3 if args.sur_static else args.seeds,
+                               args.sur_settle)
 
     if args.surrogate_only:
         print(json.dumps({"surrogate": run_surrogate()}), flush=True)

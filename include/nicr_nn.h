@@ -104,6 +104,8 @@ typedef struct nicr_sur_options {
     int32_t conserve_mass;    /* rescale O and Cr to their pre-jump totals (closed boundaries) */
     double residual_tol;      /* accept when ||R|| <= residual_tol * ||proposal - x0|| for O and Cr */
     double min_value, max_value;   /* accept only O, Cr, phi inside [min, max] */
+    int32_t settle_steps;     /* plain steps after an accepted jump before the next proposal (0 or 1: one) */
+    int32_t reserved;
 } nicr_sur_options;
 
 typedef struct nicr_sur_report {
@@ -113,21 +115,25 @@ typedef struct nicr_sur_report {
     int32_t status;           /* NICR_STEP_* of a rejected plain step, else OK */
     int32_t reserved;
     double max_residual_o, max_residual_cr;   /* largest accepted ||R|| / ||proposal - x0|| */
-    double min_rejected_residual;             /* smallest max(O, Cr) residual among rejections */
+    double min_rejected_residual;             /* smallest max(O, Cr, eta) residual among rejections */
+    double max_residual_eta;                  /* largest accepted eta residual (evolving grains) */
     nicr_step_report step;    /* the plain step's report when status != OK */
 } nicr_sur_report;
 
 /* Advance a binary64 3D model by nsteps with network jumps of `jump` steps.
- * Each cycle: one plain step x_-1 -> x0 (update u0 = x0 - x_-1); the proposal
- * xh = x0 + jump u0 + correction(x0, u0) for the state `jump` steps later (eta
- * fields are held; jump must equal the network's); optionally O and Cr
- * rescaled to their totals at x0; one plain check step from xh to xh1. The
- * jump is accepted when O, Cr, phi are inside [min, max], the model accepts the
- * check step, and for O and Cr the trapezoid residual
+ * Each cycle: plain steps ending in x_-1 -> x0 (update u0 = x0 - x_-1; one
+ * step, or settle_steps right after an accepted jump); the proposal
+ * xh = x0 + jump u0 + correction(x0, u0) of O, Cr, phi for the state `jump`
+ * steps later (jump must equal the network's); evolving eta fields jump to
+ * x0 + jump u0 with their mask rebuilt, static ones are unchanged; optionally
+ * O and Cr rescaled to their totals at x0; one plain check step from xh to
+ * xh1. The jump is accepted when O, Cr, phi (and evolving eta) are inside
+ * [min, max], the model accepts the check step, and for O, Cr (and all
+ * evolving eta fields together) the trapezoid residual
  * R = (xh - x0) - (jump / 2) (u0 + (xh1 - xh)) satisfies
- * ||R|| <= residual_tol ||xh - x0|| (a proposal that does not move fails).
- * Otherwise the model is put back exactly to x0 (fields, state index, budgets)
- * and takes the `jump` plain steps instead. */
+ * ||R|| <= residual_tol ||xh - x0|| (an O or Cr proposal that does not move
+ * fails). Otherwise the model is put back exactly to x0 (fields, eta, mask,
+ * state index, budgets) and takes the `jump` plain steps instead. */
 int nicr_sur_advance(void* net, void* model, int nsteps, const nicr_sur_options* opt, nicr_sur_report* report);
 
 #ifdef __cplusplus

@@ -318,6 +318,78 @@ __global__ __launch_bounds__(256) void k_sur_trap(long long n, const double* __r
     block_sums8<4>(v, part);
 }
 
+// Evolving grains, before the check step of a jump: the eta fields (n_eta
+// fields of ncell cells, field g at g * fs) move to the linear extrapolation
+// eta_h = eta_0 + J (eta_0 - eta_-1) in place in `cur` (eta_-1 in `prev`),
+// eta_0 goes to `bk`; the mask (S, P, gb) of `cur` goes to bk_mask and is
+// rebuilt from eta_h with the arithmetic of k_mask. part: min eta_h and
+// min(-eta_h) per block (columns 0, 1).
+__global__ __launch_bounds__(256) void k_sur_eta_pre(long long ncell, long long fs, int n_eta, double J,
+                                                     double eta_scale, const double* __restrict__ prev,
+                                                     double* __restrict__ cur, double* __restrict__ bk,
+                                                     double* __restrict__ s2, double* __restrict__ pairs,
+                                                     double* __restrict__ gb, double* __restrict__ bk_mask,
+                                                     double* __restrict__ part) {
+    double mn = INFINITY, mx = INFINITY;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ncell;
+         i += (long long)gridDim.x * blockDim.x) {
+        bk_mask[i] = s2[i];
+        bk_mask[ncell + i] = pairs[i];
+        bk_mask[2 * ncell + i] = gb[i];
+        double S = 0., pr = 0.;
+        for (int g = 0; g < n_eta; ++g) {
+            const long long k = (long long)g * fs + i;
+            const double a = prev[k], b = cur[k];
+            const double h = b + J * (b - a);
+            bk[k] = b;
+            cur[k] = h;
+            mn = fmin(mn, h);
+            mx = fmin(mx, -h);
+            const double q = h * h;
+            pr += S * q;
+            S += q;
+        }
+        const double s16 = 16. * pr;
+        s2[i] = S;
+        pairs[i] = pr;
+        gb[i] = eta_scale * (s16 < 1. ? s16 : 1.);
+    }
+    __shared__ double red[2][4];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        double x = k == 0 ? mn : mx;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x = fmin(x, __shfl_xor(x, off, 64));
+        if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const double* r = red[threadIdx.x];
+        part[blockIdx.x * 8 + threadIdx.x] = fmin(fmin(r[0], r[1]), fmin(r[2], r[3]));
+    }
+}
+
+// Trapezoid residual of the extrapolated eta fields: with d = eta_h - eta_0 =
+// J u_0, R = d - (J/2)(d / J + (eta_h1 - eta_h)); sums of R^2 and d^2 over all
+// fields (columns 0, 1).
+__global__ __launch_bounds__(256) void k_sur_eta_trap(long long ncell, long long fs, int n_eta, double J,
+                                                      const double* __restrict__ h, const double* __restrict__ h1,
+                                                      const double* __restrict__ x0, double* __restrict__ part) {
+    double v[2] = {0., 0.};
+    const double halfJ = 0.5 * J;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ncell;
+         i += (long long)gridDim.x * blockDim.x)
+        for (int g = 0; g < n_eta; ++g) {
+            const long long k = (long long)g * fs + i;
+            const double hv = h[k];
+            const double d = hv - x0[k];
+            const double r = d - halfJ * (d / J + (h1[k] - hv));
+            v[0] += r * r;
+            v[1] += d * d;
+        }
+    block_sums8<2>(v, part);
+}
+
 // acc += sum (d - J u)^2 (float arrays; binary64 sums)
 __global__ __launch_bounds__(256) void k_corr_sq(long long n, const float* __restrict__ d, const float* __restrict__ u,
                                                  double J, double* __restrict__ acc) {
@@ -562,6 +634,10 @@ struct Net {
     float *ds_in = nullptr, *ds_d = nullptr;
     // binary64 state backup of a surrogate jump (O, Cr, phi)
     double* bk[3] = {nullptr, nullptr, nullptr};
+    // evolving grains: eta before a jump (the model's field layout) and its mask S, P, gb
+    double* bk_eta = nullptr;
+    double* bk_mask = nullptr;
+    long long bk_eta_elems = 0, bk_mask_cells = 0;
     double* du[3] = {nullptr, nullptr, nullptr};   // plain update of O, Cr, phi before a jump
     double* part = nullptr;                          // block partial sums of the fused jump passes
     std::vector<double> part_host;
@@ -1512,6 +1588,36 @@ static int sur_buffers(Net* n) {
     return NICR_OK;
 }
 
+static void nfree(Net* n, void* p, size_t bytes) {
+    if (!p) return;
+    auto it = std::find(n->allocs.begin(), n->allocs.end(), p);
+    if (it != n->allocs.end()) n->allocs.erase(it);
+    hipFree(p);
+    n->bytes -= bytes;
+}
+
+// backups of the eta fields (the model's layout, padding included) and of the
+// mask for jumps over evolving grains
+static int sur_eta_buffers(Net* n, Ctx* c) {
+    const long long need = (long long)c->cfg.n_eta * c->P.fstride;
+    int rc = 0;
+    if (n->bk_eta_elems < need) {
+        nfree(n, n->bk_eta, (size_t)n->bk_eta_elems * 8);
+        n->bk_eta = nullptr;
+        n->bk_eta_elems = 0;
+        if ((rc = nalloc_t(n, &n->bk_eta, need))) return rc;
+        n->bk_eta_elems = need;
+    }
+    if (n->bk_mask_cells < c->ncell) {
+        nfree(n, n->bk_mask, (size_t)n->bk_mask_cells * 3 * 8);
+        n->bk_mask = nullptr;
+        n->bk_mask_cells = 0;
+        if ((rc = nalloc_t(n, &n->bk_mask, 3 * c->ncell))) return rc;
+        n->bk_mask_cells = c->ncell;
+    }
+    return NICR_OK;
+}
+
 // block partial sums of a fused pass -> host (columns 0..nv-1); min columns
 // (from `first_min` on) are reduced by min instead of sum
 static int sur_partials(Net* n, int nv, long double* sum, int first_min = 8) {
@@ -1538,6 +1644,10 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
     if (J != n->norm.jump) return fail(NICR_E_INVALID, "nicr_sur_advance: jump differs from the network's");
     if (!(opt->residual_tol > 0.)) return fail(NICR_E_INVALID, "nicr_sur_advance: residual_tol must be > 0");
     if ((rc = sur_buffers(n))) return rc;
+    // evolving grains: eta jumps by linear extrapolation and is checked like O and Cr
+    const bool evo = c->eta_evolves();
+    if (evo && c->te != 8) return fail(NICR_E_INVALID, "nicr_sur_advance: eta must be binary64");
+    if (evo && (rc = sur_eta_buffers(n, c))) return rc;
     const long long V = n->V(0);
     *rep = nicr_sur_report{};
     const bool conserve = opt->conserve_mass != 0;
@@ -1555,13 +1665,17 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         }
         return NICR_OK;
     };
+    const int settle = opt->settle_steps > 1 ? opt->settle_steps : 1;
+    bool after_jump = false;
     while (left > 0 && rep->status == NICR_STEP_OK) {
-        if (left < J + 2) {
+        const int pre = after_jump ? settle : 1;
+        if (left < J + 1 + pre) {
             if ((rc = plain(left))) return rc;
             break;
         }
-        // the plain step before the jump: its update u0 enters the input and the residual
-        if ((rc = plain(1))) return rc;
+        // the plain steps before the jump: the update u0 of the last enters
+        // the input and the residual
+        if ((rc = plain(pre))) return rc;
         if (rep->status != NICR_STEP_OK) break;
         DevCtl saved;
         if ((rc = sync_ctl(c, &saved))) return rc;   // also: the step has finished
@@ -1608,7 +1722,19 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         const double lo = opt->min_value, hi = opt->max_value;
         bool ok = std::isfinite(s1[0]) && std::isfinite(s1[1]) && omin >= lo && omax <= hi && cmin >= lo &&
                   cmax <= hi && pmin >= lo && pmax <= hi;
-        double ro = INFINITY, rcr = INFINITY;
+        if (evo) {
+            // eta_h and its mask in the current buffers; eta_-1 (the other
+            // buffer) is overwritten by the check step
+            const Par& P = c->P;
+            hipLaunchKernelGGL(k_sur_eta_pre, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, c->ncell, P.fstride,
+                               c->cfg.n_eta, (double)J, P.eta_scale, (const double*)c->eta[cur ^ 1],
+                               (double*)c->eta[cur], n->bk_eta, (double*)c->s2[cur], (double*)c->pairs[cur],
+                               (double*)c->gb[cur], n->bk_mask, n->part);
+            long double pe[2];
+            if ((rc = sur_partials(n, 2, pe, 0))) return rc;
+            ok = ok && (double)pe[0] >= lo && -(double)pe[1] <= hi;
+        }
+        double ro = INFINITY, rcr = INFINITY, reta = evo ? INFINITY : 0.;
         if (ok) {
             nicr_step_report rr{};
             if ((rc = nicr_step(c, 1, &rr))) return rc;
@@ -1622,24 +1748,34 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
             if ((rc = sur_partials(n, 4, pt))) return rc;
             ro = pt[1] > 0 ? (double)std::sqrt(pt[0] / pt[1]) : INFINITY;
             rcr = pt[3] > 0 ? (double)std::sqrt(pt[2] / pt[3]) : INFINITY;
-            if (!(ro <= opt->residual_tol && rcr <= opt->residual_tol)) ok = false;
+            if (evo) {
+                hipLaunchKernelGGL(k_sur_eta_trap, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, c->ncell,
+                                   c->P.fstride, c->cfg.n_eta, (double)J, (const double*)c->eta[k ^ 1],
+                                   (const double*)c->eta[k], n->bk_eta, n->part);
+                long double pr[2];
+                if ((rc = sur_partials(n, 2, pr))) return rc;
+                reta = pr[1] > 0 ? (double)std::sqrt(pr[0] / pr[1]) : (pr[0] == 0 ? 0. : INFINITY);
+            }
+            if (!(ro <= opt->residual_tol && rcr <= opt->residual_tol && reta <= opt->residual_tol)) ok = false;
         }
         if (ok) {
             ++rep->jumps_accepted;
             rep->max_residual_o = std::max(rep->max_residual_o, ro);
             rep->max_residual_cr = std::max(rep->max_residual_cr, rcr);
+            rep->max_residual_eta = std::max(rep->max_residual_eta, reta);
             rep->plain_steps += 1;
             left -= J + 1;
+            after_jump = true;
             continue;
         }
         // reject: restore the control block (state index, budgets) and the
-        // fields before the jump (eta and the mask were not touched by the
+        // fields before the jump (static eta and mask are not touched by a
         // jump; a check step wrote only the other buffers), then take the
         // plain steps
         ++rep->jumps_rejected;
-        if (std::isfinite(ro) && std::isfinite(rcr))
-            rep->min_rejected_residual = rep->jumps_rejected == 1 ? std::max(ro, rcr)
-                                                                  : std::min(rep->min_rejected_residual, std::max(ro, rcr));
+        const double rmax = std::max(std::max(ro, rcr), reta);
+        if (std::isfinite(rmax))
+            rep->min_rejected_residual = rep->jumps_rejected == 1 ? rmax : std::min(rep->min_rejected_residual, rmax);
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipMemcpyAsync(c->ctl, &saved, sizeof(saved), hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -1648,8 +1784,16 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         HIPCHK(hipMemcpyAsync(c->o[cb], n->bk[0], V * 8, hipMemcpyDeviceToDevice, n->stream));
         HIPCHK(hipMemcpyAsync(c->cr[cb], n->bk[1], V * 8, hipMemcpyDeviceToDevice, n->stream));
         HIPCHK(hipMemcpyAsync(c->p[cb], n->bk[2], V * 8, hipMemcpyDeviceToDevice, n->stream));
+        if (evo) {
+            HIPCHK(hipMemcpyAsync(c->eta[cb], n->bk_eta, (size_t)c->cfg.n_eta * c->P.fstride * 8,
+                                  hipMemcpyDeviceToDevice, n->stream));
+            HIPCHK(hipMemcpyAsync(c->s2[cb], n->bk_mask, V * 8, hipMemcpyDeviceToDevice, n->stream));
+            HIPCHK(hipMemcpyAsync(c->pairs[cb], n->bk_mask + V, V * 8, hipMemcpyDeviceToDevice, n->stream));
+            HIPCHK(hipMemcpyAsync(c->gb[cb], n->bk_mask + 2 * V, V * 8, hipMemcpyDeviceToDevice, n->stream));
+        }
         HIPCHK(hipStreamSynchronize(n->stream));
         c->refresh_valid = false;
+        after_jump = false;
         if ((rc = plain(J))) return rc;
     }
     rep->steps_done = nsteps - left;

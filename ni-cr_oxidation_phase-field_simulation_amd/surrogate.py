@@ -46,14 +46,16 @@ def make_norm(phi_mean=0.0, phi_scale=1.0, gb_mean=0.0, gb_scale=1.0, c_mean=(0.
 
 class SurOptions(ctypes.Structure):
     _fields_ = [("jump", ctypes.c_int32), ("conserve_mass", ctypes.c_int32), ("residual_tol", ctypes.c_double),
-                ("min_value", ctypes.c_double), ("max_value", ctypes.c_double)]
+                ("min_value", ctypes.c_double), ("max_value", ctypes.c_double), ("settle_steps", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class SurReport(ctypes.Structure):
     _fields_ = [("steps_done", ctypes.c_int32), ("plain_steps", ctypes.c_int32), ("jumps_accepted", ctypes.c_int32),
                 ("jumps_rejected", ctypes.c_int32), ("status", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("max_residual_o", ctypes.c_double), ("max_residual_cr", ctypes.c_double),
-                ("min_rejected_residual", ctypes.c_double), ("step", NicrStepReport)]
+                ("min_rejected_residual", ctypes.c_double), ("max_residual_eta", ctypes.c_double),
+                ("step", NicrStepReport)]
 
 
 def _declare(lib):
@@ -224,10 +226,10 @@ class UNet:
 
     # ---- surrogate stepping ----
     def advance(self, model, nsteps: int, jump: int = 50, conserve_mass: bool = True, residual_tol: float = 0.1,
-                bounds=(0.0, 1.0)) -> SurReport:
+                bounds=(0.0, 1.0), settle_steps: int = 1) -> SurReport:
         """Advance a binary64 Model by nsteps with accepted network jumps (nicr_sur_advance)."""
         opt = SurOptions(int(jump), int(bool(conserve_mass)), float(residual_tol), float(bounds[0]),
-                         float(bounds[1]))
+                         float(bounds[1]), int(settle_steps), 0)
         rep = SurReport()
         _check(self.lib, self.lib.nicr_sur_advance(self.h, model._h.h, int(nsteps), ctypes.byref(opt),
                                                    ctypes.byref(rep)))
@@ -256,30 +258,44 @@ class UNet:
 # ---- training pipeline on trajectories of the plain step ----
 
 def closed_model(nicr, shape, state, precision="fp64"):
-    """A binary64 3D Model of the surrogate workload (static grains, closed
-    boundaries, no Ckp correction, no insertion) holding `state` = (o, cr, p, gb)."""
-    cfg = nicr.make_config(3, nx=shape[0], ny=shape[1], nz=shape[2], grains=True, grain_evolution=False,
-                           bc="closed", ck="off", bounds="legacy", nucleation="off", precision=precision)
+    """A binary64 3D Model of the surrogate workload (closed boundaries, no Ckp
+    correction, no insertion) holding `state` = (o, cr, p, gb) for static
+    grains or (o, cr, p, eta[n_eta]) for evolving ones."""
+    grains = np.asarray(state[3])
+    n_eta = grains.shape[0] if grains.ndim == len(shape) + 1 else 0
+    cfg = nicr.make_config(3, nx=shape[0], ny=shape[1], nz=shape[2], grains=True, grain_evolution=n_eta > 0,
+                           n_eta=n_eta, bc="closed", ck="off", bounds="legacy", nucleation="off",
+                           precision=precision)
     m = nicr.Model(cfg)
-    for k, a in zip(("o", "cr", "p", "gb"), state):
+    for k, a in zip(("o", "cr", "p"), state[:3]):
         m.set(k, a)
+    if n_eta:
+        for g in range(n_eta):
+            m.set(g, grains[g])
+    else:
+        m.set("gb", grains)
     return m
 
 
-def record_trajectories(net, nicr, n_traj, jump=50, n_struct=16, n_seeds=16, warm=(20, 600), seed=0, log=None):
+def record_trajectories(net, nicr, n_traj, jump=50, n_struct=16, n_seeds=16, warm=(20, 600), seed=0, log=None,
+                        n_eta=0):
     """One training sample from each of n_traj plain binary64 trajectories.
 
     Trajectory t starts from the grain structure t % n_struct (Voronoi, n_seeds
-    seeds), Cr mean U(0.17, 0.23) with U(-0.005, 0.005) noise, O background
-    10^U(-3.5, -2.5) with 20 % noise, takes a random number of plain steps in
-    `warm`, then records (state, plain update, change over `jump` steps)."""
+    seeds; static GB mask, or n_eta > 0 evolving eta fields), Cr mean
+    U(0.17, 0.23) with U(-0.005, 0.005) noise, O background 10^U(-3.5, -2.5)
+    with 20 % noise, takes a random number of plain steps in `warm`, then
+    records (state, plain update, change over `jump` steps)."""
     from . import synthetic as syn
     shape = net.shape
     t0 = time.perf_counter()
     gbs = []
     for s in range(n_struct):
-        _, dist, _ = syn.voronoi_labels(shape, n_seeds, seed=seed + 100 + s)
-        gbs.append(syn.gb_mask_from_distance(dist))
+        labels, dist, _ = syn.voronoi_labels(shape, n_seeds, seed=seed + 100 + s)
+        if n_eta > 0:
+            gbs.append(syn.eta_from_labels(labels, dist, syn.colour_seeds(labels, n_seeds, n_eta), n_eta))
+        else:
+            gbs.append(syn.gb_mask_from_distance(dist))
     t1 = time.perf_counter()
     steps = 0
     for t in range(n_traj):
@@ -295,6 +311,8 @@ def record_trajectories(net, nicr, n_traj, jump=50, n_struct=16, n_seeds=16, war
             raise RuntimeError(f"trajectory {t}: a plain step was rejected (status {rr.status})")
         steps += w + 1 + jump
         m.close()
+        if log and (t + 1) % 50 == 0 and t + 1 < n_traj:
+            log(f"{t + 1} trajectories recorded ({time.perf_counter() - t1:.1f} s)")
     t2 = time.perf_counter()
     if log:
         log(f"{n_struct} grain structures in {t1 - t0:.1f} s; {n_traj} trajectories ({steps} plain steps) "

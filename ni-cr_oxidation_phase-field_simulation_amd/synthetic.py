@@ -153,16 +153,22 @@ def eta_from_labels(labels, dist, colour, n_eta, ell=1.5, metal=None):
     return eta
 
 
-def closed_alloy(shape, n_seeds=16, seed=1, cr_mean=0.20, cr_noise=0.005, o_background=1e-3, o_noise=0.2):
-    """State of the surrogate workloads: static Voronoi grains (GB mask of the
-    fixed-grain mode), Cr = cr_mean + U(-cr_noise, cr_noise), O = o_background
-    (1 + U(-o_noise, o_noise)), no oxide (phi = 0), no inlet layer. With closed
-    boundaries the step conserves O and Cr and no cell leaves [0, 1].
-    Returns (o, cr, p, gb)."""
+def closed_alloy(shape, n_seeds=16, seed=1, cr_mean=0.20, cr_noise=0.005, o_background=1e-3, o_noise=0.2,
+                 n_eta=0):
+    """State of the surrogate workloads: Voronoi grains, Cr = cr_mean +
+    U(-cr_noise, cr_noise), O = o_background (1 + U(-o_noise, o_noise)), no
+    oxide (phi = 0), no inlet layer. With closed boundaries the step conserves
+    O and Cr and no cell leaves [0, 1]. n_eta = 0: static grains, returns
+    (o, cr, p, gb) with the GB mask of the fixed-grain mode; n_eta > 0: grains
+    coloured onto n_eta evolving eta fields, returns (o, cr, p, eta[n_eta])."""
     rng = np.random.default_rng(seed)
-    _, dist, _ = voronoi_labels(shape, n_seeds, seed=seed)
-    gb = gb_mask_from_distance(dist)
+    labels, dist, _ = voronoi_labels(shape, n_seeds, seed=seed)
+    if n_eta > 0:
+        grains = eta_from_labels(labels, dist, colour_seeds(labels, n_seeds, n_eta), n_eta)
+    else:
+        grains = gb_mask_from_distance(dist)
+    del labels, dist
     cr = cr_mean + rng.uniform(-cr_noise, cr_noise, size=shape)
     o = o_background * (1.0 + rng.uniform(-o_noise, o_noise, size=shape))
     p = np.zeros(shape)
-    return o, cr, p, gb
+    return o, cr, p, grains

@@ -187,34 +187,71 @@ def test_inference_network_equals_training_network():
     b.close()
 
 
-def _closed_model(nicr, shape, seed, **kw):
-    o, cr, p, gb = _syn().closed_alloy(shape, n_seeds=8, seed=seed)
-    cfg = nicr.make_config(3, nx=shape[0], ny=shape[1], nz=shape[2], grains=True, grain_evolution=False, bc="closed",
-                           ck="off", bounds="legacy", nucleation="off", **kw)
-    m = nicr.Model(cfg)
-    for k, a in zip(("o", "cr", "p", "gb"), (o, cr, p, gb)):
-        m.set(k, a)
-    return m
+def _closed_model(nicr, shape, seed, n_eta=0):
+    state = _syn().closed_alloy(shape, n_seeds=8, seed=seed, n_eta=n_eta)
+    return _sur().closed_model(nicr, shape, state)
 
 
-def test_rejected_jumps_equal_plain_steps(nicr):
-    """With a residual tolerance no proposal meets, every jump is rejected and
-    the model equals plain stepping bitwise (fields and budgets)."""
-    shape = (16, 16, 24)
-    a = _closed_model(nicr, shape, 4)
-    b = _closed_model(nicr, shape, 4)
-    net = _sur().UNet(shape, seed=1)
+def _plain_norm(net, J):
     net.set_norm(gb_mean=0.5, gb_scale=0.3, c_mean=[-3.5, 0.2, -7.0, -6.0], c_scale=[0.5, 0.02, 0.5, 0.5],
-                 gain=[0.05, 0.05, 0.0], jump=20)
+                 gain=[0.05, 0.05, 0.0], jump=J)
+
+
+@pytest.mark.parametrize("n_eta", [0, 8], ids=["static", "evolving"])
+def test_rejected_jumps_equal_plain_steps(nicr, n_eta):
+    """With a residual tolerance no proposal meets, every jump is rejected and
+    the model equals plain stepping bitwise (fields, eta, mask and budgets)."""
+    shape = (16, 16, 24)
+    a = _closed_model(nicr, shape, 4, n_eta)
+    b = _closed_model(nicr, shape, 4, n_eta)
+    net = _sur().UNet(shape, seed=1)
+    _plain_norm(net, 20)
     rep = net.advance(a, 130, jump=20, residual_tol=1e-12)
     assert rep.status == 0 and rep.steps_done == 130
     assert rep.jumps_accepted == 0 and rep.jumps_rejected >= 5
     assert rep.plain_steps == 130
     rb = b.step(130)
     assert rb.steps_done == 130
-    for f in ("o", "cr", "p"):
+    for f in ("o", "cr", "p", "gb") + tuple(range(n_eta)):
         assert np.array_equal(a.get(f), b.get(f)), f
     assert a.b == b.b
+    # the next steps read the restored mask
+    a.step(3)
+    b.step(3)
+    for f in ("o", "cr", "p") + tuple(range(n_eta)):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    net.close()
+
+
+def test_evolving_eta_jump_is_checked_extrapolation(nicr):
+    """Evolving grains: an accepted jump moves every eta field to the linear
+    extrapolation x0 + J (x0 - x_-1) with its mask rebuilt, then takes the check
+    step. With a zero network head the O, Cr, phi proposal is the same
+    extrapolation, so the result equals the extrapolated state uploaded to a
+    twin model and stepped once, bitwise; the eta residual is reported."""
+    J = 20
+    shape = (16, 16, 24)
+    n_eta = 8
+    a = _closed_model(nicr, shape, 5, n_eta)
+    b = _closed_model(nicr, shape, 5, n_eta)
+    a.step(30)
+    b.step(30)
+    names = ("o", "cr", "p") + tuple(range(n_eta))
+    xm = {f: b.get(f) for f in names}
+    b.step(1)
+    x0 = {f: b.get(f) for f in names}
+    for f in names:
+        b.set(f, x0[f] + J * (x0[f] - xm[f]))
+    rb = b.step(1)
+    assert rb.steps_done == 1
+    net = _sur().UNet(shape, seed=1)
+    _plain_norm(net, J)
+    rep = net.advance(a, J + 2, jump=J, conserve_mass=False, residual_tol=1e30, bounds=(-1.0, 2.0))
+    assert rep.status == 0 and rep.steps_done == J + 2
+    assert rep.jumps_accepted == 1 and rep.jumps_rejected == 0 and rep.plain_steps == 2
+    assert 0.0 < rep.max_residual_eta < 1e30
+    for f in names + ("gb",):
+        assert np.array_equal(a.get(f), b.get(f)), f
     net.close()
 
 
