@@ -194,7 +194,7 @@ def unet_flops_per_cell():
     return sum(2 * 27 * ci * co / 8 ** lv for lv, convs in levels.items() for ci, co in convs)
 
 
-def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, warm, tol, seed, n_eta, n_seeds, settle):
+def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, warm, tol, seed, n_eta, n_seeds, settle, noise):
     """Learned surrogate on the closed workload: training set from n_traj
     plain binary64 trajectories at n_train^3, Adam on the GPU, then n_run^3 from
     one state (after `warm` plain steps) advanced nsteps by the plain binary64
@@ -203,7 +203,8 @@ def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, warm, tol, seed
     plain run's.
     n_eta > 0: n_seeds grains at n_run^3 coloured onto n_eta evolving eta fields
     (training volumes hold the same grain size); n_eta = 0: static grains.
-    settle: plain steps after an accepted jump before the next proposal."""
+    settle: plain steps after an accepted jump before the next proposal;
+    noise: largest relative perturbation of the recorded samples."""
     sur = importlib.import_module(PKG_NAME + ".surrogate")
     syn = importlib.import_module(PKG_NAME + ".synthetic")
 
@@ -213,7 +214,8 @@ def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, warm, tol, seed
     J = 50
     train_seeds = max(2, round(n_seeds * (n_train / n_run) ** 3))
     net = sur.UNet((n_train,) * 3, max_batch=2, train=True, max_samples=n_traj, seed=seed)
-    rec = sur.record_trajectories(net, nicr, n_traj, jump=J, n_seeds=train_seeds, seed=seed, log=log, n_eta=n_eta)
+    rec = sur.record_trajectories(net, nicr, n_traj, jump=J, n_seeds=train_seeds, seed=seed, log=log, n_eta=n_eta,
+                                  noise=noise)
     net.fit_norm()
     tr = sur.train_schedule(net, iters, batch=2, lr=1e-3, seed=seed, log=log)
     params, norm = net.params(), net.norm()
@@ -304,7 +306,7 @@ def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, warm, tol, seed
         "forward_roofline": {"bound": "mfma", "achieved": round(flops / (fwd_ms * 1e-3) / 1e12, 1), "peak": 2500.0,
                              "unit": "TFLOP/s", "frac": round(flops / (fwd_ms * 1e-3) / 1e12 / 2500.0, 4),
                              "flops_per_forward": flops},
-        "training": {"volume": f"{n_train}^3", "trajectories": rec["trajectories"],
+        "training": {"volume": f"{n_train}^3", "trajectories": rec["trajectories"], "noise": noise,
                      "plain_steps": rec["plain_steps"], "record_s": round(rec["seconds"], 1),
                      "iters": tr["iters"], "batch": tr["batch"], "train_s": round(tr["seconds"], 1),
                      "loss_first": round(tr["loss_first"], 4), "loss_last": round(tr["loss_last"], 4)},
@@ -335,8 +337,10 @@ def main():
     ap.add_argument("--sur-run-n", type=int, default=256, help="side of the surrogate run")
     ap.add_argument("--sur-steps", type=int, default=5000, help="steps of the timed run (5000: the 256^3 run length)")
     ap.add_argument("--sur-warm", type=int, default=0, help="plain steps before the timed run")
-    ap.add_argument("--sur-settle", type=int, default=5,
+    ap.add_argument("--sur-settle", type=int, default=1,
                     help="plain steps after an accepted jump before the next proposal")
+    ap.add_argument("--sur-noise", type=float, default=0.2,
+                    help="largest perturbation of the recorded training states (x jump x RMS update)")
     ap.add_argument("--sur-tol", type=float, default=0.1, help="trapezoid residual tolerance of a jump")
     ap.add_argument("--sur-static", action="store_true",
                     help="surrogate on static grains (128 seeds at 256^3) instead of the --eta evolving fields")
@@ -358,7 +362,7 @@ def main():
         return surrogate_bench(nicr, args.sur_train_n, args.sur_run_n, args.sur_traj, args.sur_iters,
                                args.sur_steps, args.sur_warm, args.sur_tol, 20260909, 0 if args.sur_static else args.eta,
                                128 * (args.sur_run_n // 256) ** 3 if args.sur_static else args.seeds,
-                               args.sur_settle)
+                               args.sur_settle, args.sur_noise)
 
     if args.surrogate_only:
         print(json.dumps({"surrogate": run_surrogate()}), flush=True)

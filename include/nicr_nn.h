@@ -117,6 +117,8 @@ typedef struct nicr_sur_report {
     double max_residual_o, max_residual_cr;   /* largest accepted ||R|| / ||proposal - x0|| */
     double min_rejected_residual;             /* smallest max(O, Cr, eta) residual among rejections */
     double max_residual_eta;                  /* largest accepted eta residual (evolving grains) */
+    double last_residual[3];                  /* O, Cr, eta residuals of the last jump attempt (inf: not
+                                                 computed, the proposal failed the bounds or the check step) */
     nicr_step_report step;    /* the plain step's report when status != OK */
 } nicr_sur_report;
 

@@ -1758,6 +1758,9 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
             }
             if (!(ro <= opt->residual_tol && rcr <= opt->residual_tol && reta <= opt->residual_tol)) ok = false;
         }
+        rep->last_residual[0] = ro;
+        rep->last_residual[1] = rcr;
+        rep->last_residual[2] = reta;
         if (ok) {
             ++rep->jumps_accepted;
             rep->max_residual_o = std::max(rep->max_residual_o, ro);

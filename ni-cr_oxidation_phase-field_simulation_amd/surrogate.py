@@ -55,6 +55,7 @@ class SurReport(ctypes.Structure):
                 ("jumps_rejected", ctypes.c_int32), ("status", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("max_residual_o", ctypes.c_double), ("max_residual_cr", ctypes.c_double),
                 ("min_rejected_residual", ctypes.c_double), ("max_residual_eta", ctypes.c_double),
+                ("last_residual", ctypes.c_double * 3),
                 ("step", NicrStepReport)]
 
 
@@ -278,14 +279,21 @@ def closed_model(nicr, shape, state, precision="fp64"):
 
 
 def record_trajectories(net, nicr, n_traj, jump=50, n_struct=16, n_seeds=16, warm=(20, 600), seed=0, log=None,
-                        n_eta=0):
+                        n_eta=0, noise=0.0, noise_frac=0.75):
     """One training sample from each of n_traj plain binary64 trajectories.
 
     Trajectory t starts from the grain structure t % n_struct (Voronoi, n_seeds
     seeds; static GB mask, or n_eta > 0 evolving eta fields), Cr mean
     U(0.17, 0.23) with U(-0.005, 0.005) noise, O background 10^U(-3.5, -2.5)
     with 20 % noise, takes a random number of plain steps in `warm`, then
-    records (state, plain update, change over `jump` steps)."""
+    records (state, plain update, change over `jump` steps).
+
+    noise > 0: in a share noise_frac of the trajectories, O and Cr are then
+    perturbed cell by cell with Gaussian noise of RMS c * jump * RMS(u), c ~
+    U(0, noise), u the field's last plain update, before the sample is
+    recorded. A jump leaves errors of this kind in the state, and the plain
+    steps of the sample show how the method damps them; without such samples
+    the extrapolation jump * u amplifies the stiff ones from jump to jump."""
     from . import synthetic as syn
     shape = net.shape
     t0 = time.perf_counter()
@@ -306,6 +314,15 @@ def record_trajectories(net, nicr, n_traj, jump=50, n_struct=16, n_seeds=16, war
         m = closed_model(nicr, shape, (o, cr, np.zeros(shape), gbs[t % n_struct]))
         w = int(r.integers(warm[0], warm[1]))
         rep = m.step(w)
+        if noise > 0.0 and r.uniform() < noise_frac:
+            c = r.uniform(0.0, noise)
+            before = {k: m.get(k) for k in ("o", "cr")}
+            m.step(1)
+            steps += 1
+            for k in ("o", "cr"):
+                a = m.get(k)
+                amp = c * jump * float(np.sqrt(np.mean((a - before[k]) ** 2)))
+                m.set(k, np.clip(a + amp * r.standard_normal(shape), 0.0, 1.0))
         rr = net.record(m, jump)
         if rr.status != 0:
             raise RuntimeError(f"trajectory {t}: a plain step was rejected (status {rr.status})")
