@@ -255,13 +255,17 @@ def test_evolving_eta_jump_is_checked_extrapolation(nicr):
     net.close()
 
 
-def test_training_reduces_loss_and_jumps_are_checked(nicr):
+@pytest.mark.parametrize("n_eta", [0, 8], ids=["static", "evolving"])
+def test_training_reduces_loss_and_jumps_are_checked(nicr, n_eta):
     """Samples from plain trajectories, Adam on the GPU: the training loss falls;
-    accepted jumps keep O and Cr totals and stay close to plain stepping."""
+    accepted jumps keep O and Cr totals and stay close to plain stepping; the
+    trained network's jumps are markedly more accurate than the same jumps
+    without it (a zero head: pure linear extrapolation J u), measured by the
+    trapezoid residuals that decide acceptance."""
     shape = (32, 32, 32)
     net = _sur().UNet(shape, max_batch=2, train=True, max_samples=12, seed=3)
     for t in range(4):
-        m = _closed_model(nicr, shape, 10 + t)
+        m = _closed_model(nicr, shape, 10 + t, n_eta)
         m.step(40)
         for k in range(3):
             rep = net.record(m, 20)
@@ -276,8 +280,8 @@ def test_training_reduces_loss_and_jumps_are_checked(nicr):
     assert np.all(np.isfinite(losses))
     assert last < 0.5 * first
     # surrogate stepping from a fresh state of the same generator
-    a = _closed_model(nicr, shape, 99)
-    b = _closed_model(nicr, shape, 99)
+    a = _closed_model(nicr, shape, 99, n_eta)
+    b = _closed_model(nicr, shape, 99, n_eta)
     a.step(120)
     b.step(120)
     m0 = [sum(a.stats("o")[0]), sum(a.stats("cr")[0])]
@@ -294,4 +298,16 @@ def test_training_reduces_loss_and_jumps_are_checked(nicr):
         rel = np.linalg.norm(fa - fb) / np.linalg.norm(fb)
         print(f"{f}: rel L2 vs plain {rel:.3e}")
         assert rel < 1e-2
+    # the same jumps without the network: a zero head proposes x0 + J u0
+    zero = _sur().UNet(shape, seed=1)
+    zero.set_norm(net.norm())
+    z = _closed_model(nicr, shape, 99, n_eta)
+    z.step(120)
+    rz = zero.advance(z, 200, jump=20, residual_tol=0.3)
+    print(f"largest accepted residuals: trained O {rep.max_residual_o:.3f} Cr {rep.max_residual_cr:.3f}; "
+          f"without the network O {rz.max_residual_o:.3f} Cr {rz.max_residual_cr:.3f}")
+    assert rep.jumps_accepted >= 8 and rz.jumps_accepted >= 1
+    assert rep.max_residual_o < 0.6 * rz.max_residual_o
+    assert rep.max_residual_cr < 0.6 * rz.max_residual_cr
+    zero.close()
     net.close()
