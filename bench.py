@@ -198,12 +198,12 @@ def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, warm, tol, seed
     """Learned surrogate on the closed workload: training set from n_traj
     plain binary64 trajectories at n_train^3, Adam on the GPU, then n_run^3 from
     one state (after `warm` plain steps) advanced nsteps by the plain binary64
-    step and by surrogate jumps of 50 steps (in logged chunks of 520 steps);
+    step and by surrogate jumps of 50 steps (in logged chunks of 511 steps);
     both timed, the surrogate's final fields and O/Cr totals compared with the
     plain run's.
     n_eta > 0: n_seeds grains at n_run^3 coloured onto n_eta evolving eta fields
     (training volumes hold the same grain size); n_eta = 0: static grains.
-    settle: plain steps after an accepted jump before the next proposal;
+    settle: extra plain steps between a jump's check step and the next proposal;
     noise: largest relative perturbation of the recorded samples."""
     sur = importlib.import_module(PKG_NAME + ".surrogate")
     syn = importlib.import_module(PKG_NAME + ".synthetic")
@@ -244,7 +244,7 @@ def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, warm, tol, seed
     t_plain = time.perf_counter() - t0
     log(f"plain binary64: {nsteps} steps in {t_plain:.2f} s")
     tot = {"steps": 0, "acc": 0, "rej": 0, "plain": 0, "res": [0.0, 0.0, 0.0], "min_rej": None}
-    chunk = 10 * (J + 2)
+    chunk = 1 + 10 * (J + 1)   # one plain step, then ten jumps with their check steps
     t_sur = 0.0
     while tot["steps"] < nsteps:
         k = min(chunk, nsteps - tot["steps"])
@@ -337,8 +337,8 @@ def main():
     ap.add_argument("--sur-run-n", type=int, default=256, help="side of the surrogate run")
     ap.add_argument("--sur-steps", type=int, default=5000, help="steps of the timed run (5000: the 256^3 run length)")
     ap.add_argument("--sur-warm", type=int, default=0, help="plain steps before the timed run")
-    ap.add_argument("--sur-settle", type=int, default=1,
-                    help="plain steps after an accepted jump before the next proposal")
+    ap.add_argument("--sur-settle", type=int, default=0,
+                    help="extra plain steps between a jump's check step and the next proposal")
     ap.add_argument("--sur-noise", type=float, default=0.2,
                     help="largest perturbation of the recorded training states (x jump x RMS update)")
     ap.add_argument("--sur-tol", type=float, default=0.1, help="trapezoid residual tolerance of a jump")

@@ -104,7 +104,7 @@ typedef struct nicr_sur_options {
     int32_t conserve_mass;    /* rescale O and Cr to their pre-jump totals (closed boundaries) */
     double residual_tol;      /* accept when ||R|| <= residual_tol * ||proposal - x0|| for O and Cr */
     double min_value, max_value;   /* accept only O, Cr, phi inside [min, max] */
-    int32_t settle_steps;     /* plain steps after an accepted jump before the next proposal (0 or 1: one) */
+    int32_t settle_steps;     /* extra plain steps before a proposal that follows other steps (0: none) */
     int32_t reserved;
 } nicr_sur_options;
 
@@ -123,8 +123,10 @@ typedef struct nicr_sur_report {
 } nicr_sur_report;
 
 /* Advance a binary64 3D model by nsteps with network jumps of `jump` steps.
- * Each cycle: plain steps ending in x_-1 -> x0 (update u0 = x0 - x_-1; one
- * step, or settle_steps right after an accepted jump); the proposal
+ * Each cycle starts from two consecutive states x_-1 -> x0 (update u0 =
+ * x0 - x_-1): the first cycle takes one plain step for them; later cycles use
+ * the last step taken (the check step of an accepted jump or the last plain
+ * step of a rejected one) after settle_steps more plain steps; the proposal
  * xh = x0 + jump u0 + correction(x0, u0) of O, Cr, phi for the state `jump`
  * steps later (jump must equal the network's); evolving eta fields jump to
  * x0 + jump u0 with their mask rebuilt, static ones are unchanged; optionally

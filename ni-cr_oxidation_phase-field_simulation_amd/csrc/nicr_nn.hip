@@ -1665,16 +1665,18 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         }
         return NICR_OK;
     };
-    const int settle = opt->settle_steps > 1 ? opt->settle_steps : 1;
-    bool after_jump = false;
+    if (opt->settle_steps < 0) return fail(NICR_E_INVALID, "nicr_sur_advance: settle_steps must be >= 0");
+    // the update u0 of a jump is the difference of the two state buffers, so
+    // they must hold consecutive states: true after the check step of an
+    // accepted jump and after plain steps, not on entry
+    bool consecutive = false;
     while (left > 0 && rep->status == NICR_STEP_OK) {
-        const int pre = after_jump ? settle : 1;
+        const int pre = consecutive ? opt->settle_steps : 1;
         if (left < J + 1 + pre) {
             if ((rc = plain(left))) return rc;
             break;
         }
-        // the plain steps before the jump: the update u0 of the last enters
-        // the input and the residual
+        // plain steps before the jump (none when the last step already gives u0)
         if ((rc = plain(pre))) return rc;
         if (rep->status != NICR_STEP_OK) break;
         DevCtl saved;
@@ -1768,7 +1770,7 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
             rep->max_residual_eta = std::max(rep->max_residual_eta, reta);
             rep->plain_steps += 1;
             left -= J + 1;
-            after_jump = true;
+            consecutive = true;
             continue;
         }
         // reject: restore the control block (state index, budgets) and the
@@ -1796,8 +1798,8 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         }
         HIPCHK(hipStreamSynchronize(n->stream));
         c->refresh_valid = false;
-        after_jump = false;
         if ((rc = plain(J))) return rc;
+        consecutive = true;
     }
     rep->steps_done = nsteps - left;
     return NICR_OK;

@@ -227,7 +227,7 @@ class UNet:
 
     # ---- surrogate stepping ----
     def advance(self, model, nsteps: int, jump: int = 50, conserve_mass: bool = True, residual_tol: float = 0.1,
-                bounds=(0.0, 1.0), settle_steps: int = 1) -> SurReport:
+                bounds=(0.0, 1.0), settle_steps: int = 0) -> SurReport:
         """Advance a binary64 Model by nsteps with accepted network jumps (nicr_sur_advance)."""
         opt = SurOptions(int(jump), int(bool(conserve_mass)), float(residual_tol), float(bounds[0]),
                          float(bounds[1]), int(settle_steps), 0)
