@@ -181,18 +181,6 @@ __global__ void k_target(int V, const float* __restrict__ d, const float* __rest
     *reinterpret_cast<float4*>(tgt + 4LL * p) = make_float4(t[0], t[1], t[2], 0.f);
 }
 
-// Surrogate jump: field_c += J u_c + out[p][c] * out_scale_c (binary64 state).
-__global__ void k_apply_jump(int V, const float* __restrict__ out, VolNorm vn, const double* __restrict__ uo,
-                             const double* __restrict__ uc, const double* __restrict__ up, double* __restrict__ o,
-                             double* __restrict__ cr, double* __restrict__ ph) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= V) return;
-    const float4 v = *reinterpret_cast<const float4*>(out + 4LL * p);
-    o[p] += vn.jump * uo[p] + (double)v.x * vn.out_scale[0];
-    cr[p] += vn.jump * uc[p] + (double)v.y * vn.out_scale[1];
-    ph[p] += vn.jump * up[p] + (double)v.z * vn.out_scale[2];
-}
-
 // ---- fused passes of a surrogate jump (binary64 state, block partial sums) ----
 
 constexpr int SUR_BLOCKS = 1024;   // blocks of the fused passes; partial sums: SUR_BLOCKS x 8
@@ -415,67 +403,6 @@ __global__ __launch_bounds__(256) void k_moments_d(long long n, const double* __
         const double v = a[i];
         s += v;
         t += v * v;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        s += __shfl_xor(s, o, 64);
-        t += __shfl_xor(t, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x >> 6] = s;
-        red[1][threadIdx.x >> 6] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        atomicAdd(&acc[0], red[0][0] + red[0][1] + red[0][2] + red[0][3]);
-        atomicAdd(&acc[1], red[1][0] + red[1][1] + red[1][2] + red[1][3]);
-    }
-}
-
-__global__ void k_scale(long long n, double* __restrict__ a, double f) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] *= f;
-}
-
-// acc[0] += sum (a - b)^2, acc[1] += sum a^2 (binary64).
-__global__ __launch_bounds__(256) void k_diff_sq(long long n, const double* __restrict__ a,
-                                                 const double* __restrict__ b, double* __restrict__ acc) {
-    __shared__ double red[2][4];
-    double s = 0., t = 0.;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const double d = a[i] - b[i];
-        s += d * d;
-        t += a[i] * a[i];
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        s += __shfl_xor(s, o, 64);
-        t += __shfl_xor(t, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x >> 6] = s;
-        red[1][threadIdx.x >> 6] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        atomicAdd(&acc[0], red[0][0] + red[0][1] + red[0][2] + red[0][3]);
-        atomicAdd(&acc[1], red[1][0] + red[1][1] + red[1][2] + red[1][3]);
-    }
-}
-
-// Trapezoid residual of a jump of J steps from x0 to xh: R = (xh - x0) -
-// (J/2)(u0 + (xh1 - xh)) with u0 the plain update at x0 and xh1 one plain step
-// from xh. acc[0] += sum R^2, acc[1] += sum (xh - x0)^2.
-__global__ __launch_bounds__(256) void k_trap(long long n, const double* __restrict__ xh,
-                                              const double* __restrict__ xh1, const double* __restrict__ x0,
-                                              const double* __restrict__ u0, double halfJ, double* __restrict__ acc) {
-    __shared__ double red[2][4];
-    double s = 0., t = 0.;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const double d = xh[i] - x0[i];
-        const double r = d - halfJ * (u0[i] + (xh1[i] - xh[i]));
-        s += r * r;
-        t += d * d;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
