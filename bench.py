@@ -436,8 +436,19 @@ def main():
         wall64, kt64 = time_steps(m64, max(10, args.steps // 4), 2, "fp64")
         n64 = max(10, args.steps // 4)
         m64.close()
-        fp64 = {"value": cells * n64 / wall64 / 1e9, "ms_per_step": wall64 / n64 * 1e3,
-                "kernel_ms": {k: v[0] for k, v in kt64.items()}}
+        ms64 = wall64 / n64 * 1e3
+        alg64 = (3 + args.eta) * 8 * 2 * cells   # o, cr, phi and eta read and written once, binary64
+        fp64 = {"value": cells * n64 / wall64 / 1e9, "ms_per_step": ms64,
+                "kernel_ms": {k: v[0] for k, v in kt64.items()},
+                "algorithmic_bytes_per_step": alg64, "algorithmic_GBs": round(alg64 / ms64 / 1e6, 1),
+                "hbm_fraction": round(alg64 / ms64 / 1e6 / HBM_PEAK_GBS, 4)}
+        path64 = os.path.join(REPO, "profiles", "traffic_fp64_step_256cube_32eta.json")
+        if shape == (256, 256, 256) and args.eta == 32 and os.path.exists(path64):
+            with open(path64) as f:
+                t64 = json.load(f)
+            fp64["dram"] = {"bytes_per_step": t64["hbm_bytes_per_step"],
+                            "GBs": round(t64["hbm_bytes_per_step"] / ms64 / 1e6, 1),
+                            "source": os.path.relpath(path64, REPO)}
     del state
 
     # Roofline of the dominant kernel, k_eta_march (one launch per pass of eta
