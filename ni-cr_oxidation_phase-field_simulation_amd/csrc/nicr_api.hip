@@ -72,7 +72,8 @@ struct Ctx {
     int* nuc_centres = nullptr;
     int g_trial = 0, g_eta = 0, g_stat = 0;
     int partial_slots = 0;         // budget partial slots allocated (one per block of the trial kernel)
-    int pde_ty = 4;                // Y tile height of the marching PDE kernel (NICR_PDE_TY: 2, 4, 6, 8)
+    int pde_ty = 0;                // Y tile height of the marching PDE kernel (NICR_PDE_TY: 2, 4, 6, 8;
+                                   // 0: 8 in binary64, 4 in mixed precision, the faster of each at 256^3)
     int pde_nseg = 0;              // X segments of the marching PDE kernel (NICR_PDE_NSEG; 0: auto)
     int cur = 0;                 // host mirror of ctl->cur
     bool mask_dirty = true;
@@ -565,6 +566,7 @@ int nicr_create(const nicr_config* cfg, void** out) {
     c->prec = c->cfg.precision;
     c->te = c->prec == PREC_FP64 ? 8 : 4;
     c->tm = c->te;
+    if (c->pde_ty == 0) c->pde_ty = c->prec == PREC_FP64 ? 8 : 4;
     const int nx = c->cfg.nx, ny = c->cfg.ny, nz = c->cfg.nz;
     c->ncell = (long long)nx * ny * nz;
     Par& P = c->P;
