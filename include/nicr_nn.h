@@ -105,7 +105,7 @@ typedef struct nicr_sur_options {
     double residual_tol;      /* accept when ||R|| <= residual_tol * ||proposal - x0|| for O and Cr */
     double min_value, max_value;   /* accept only O, Cr, phi inside [min, max] */
     int32_t settle_steps;     /* extra plain steps before a proposal that follows other steps (0: none) */
-    int32_t reserved;
+    int32_t fallback_steps;   /* plain steps after a rejected jump before the next proposal (0: jump) */
 } nicr_sur_options;
 
 typedef struct nicr_sur_report {
@@ -137,7 +137,8 @@ typedef struct nicr_sur_report {
  * R = (xh - x0) - (jump / 2) (u0 + (xh1 - xh)) satisfies
  * ||R|| <= residual_tol ||xh - x0|| (an O or Cr proposal that does not move
  * fails). Otherwise the model is put back exactly to x0 (fields, eta, mask,
- * state index, budgets) and takes the `jump` plain steps instead. */
+ * state index, budgets) and takes fallback_steps (default `jump`) plain steps
+ * instead. */
 int nicr_sur_advance(void* net, void* model, int nsteps, const nicr_sur_options* opt, nicr_sur_report* report);
 
 #ifdef __cplusplus

@@ -1593,6 +1593,8 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         return NICR_OK;
     };
     if (opt->settle_steps < 0) return fail(NICR_E_INVALID, "nicr_sur_advance: settle_steps must be >= 0");
+    if (opt->fallback_steps < 0) return fail(NICR_E_INVALID, "nicr_sur_advance: fallback_steps must be >= 0");
+    const int fallback = opt->fallback_steps > 0 ? opt->fallback_steps : J;
     // the update u0 of a jump is the difference of the two state buffers, so
     // they must hold consecutive states: true after the check step of an
     // accepted jump and after plain steps, not on entry
@@ -1725,7 +1727,7 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         }
         HIPCHK(hipStreamSynchronize(n->stream));
         c->refresh_valid = false;
-        if ((rc = plain(J))) return rc;
+        if ((rc = plain(std::min(fallback, left)))) return rc;
         consecutive = true;
     }
     rep->steps_done = nsteps - left;

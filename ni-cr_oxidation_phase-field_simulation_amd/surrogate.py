@@ -47,7 +47,7 @@ def make_norm(phi_mean=0.0, phi_scale=1.0, gb_mean=0.0, gb_scale=1.0, c_mean=(0.
 class SurOptions(ctypes.Structure):
     _fields_ = [("jump", ctypes.c_int32), ("conserve_mass", ctypes.c_int32), ("residual_tol", ctypes.c_double),
                 ("min_value", ctypes.c_double), ("max_value", ctypes.c_double), ("settle_steps", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("fallback_steps", ctypes.c_int32)]
 
 
 class SurReport(ctypes.Structure):
@@ -227,10 +227,10 @@ class UNet:
 
     # ---- surrogate stepping ----
     def advance(self, model, nsteps: int, jump: int = 50, conserve_mass: bool = True, residual_tol: float = 0.1,
-                bounds=(0.0, 1.0), settle_steps: int = 0) -> SurReport:
+                bounds=(0.0, 1.0), settle_steps: int = 0, fallback_steps: int = 0) -> SurReport:
         """Advance a binary64 Model by nsteps with accepted network jumps (nicr_sur_advance)."""
         opt = SurOptions(int(jump), int(bool(conserve_mass)), float(residual_tol), float(bounds[0]),
-                         float(bounds[1]), int(settle_steps), 0)
+                         float(bounds[1]), int(settle_steps), int(fallback_steps))
         rep = SurReport()
         _check(self.lib, self.lib.nicr_sur_advance(self.h, model._h.h, int(nsteps), ctypes.byref(opt),
                                                    ctypes.byref(rep)))
