@@ -7,9 +7,10 @@
 // instance gives bitwise the results of its own nicr context. One ensemble step
 // is two launches over all instances:
 //   k_batch_pde     a persistent grid over a flat list of tiles (TX rows of x by
-//                   64 cells of y, of one instance): o, cr, phi, gb of the tile
-//                   and its halo into LDS, mu and M over the tile and a one-cell
-//                   halo, then the trial of the tile; budget partials per tile,
+//                   64 cells of y, of one instance), a contiguous run of tiles
+//                   per block: o, cr, phi, gb of the tile and its halo into LDS,
+//                   mu and M over the tile and a one-cell halo, then the trial of
+//                   the tile; budget partials per run of one instance's tiles,
 //                   first failing cell per instance;
 //   k_batch_commit  one block per instance: that instance's all-or-nothing
 //                   acceptance (k_commit per instance).
@@ -66,8 +67,30 @@ __global__ __launch_bounds__(64 * TX) void k_batch_pde(Par P, BBufs B, int ntile
     double* sg = sp + L::P1;                                     // [P1] gb
     const int tid = threadIdx.x;
     const int tx = tid >> 6, ty = tid & 63;
-    for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+    // A block takes a contiguous run of tiles (the list is grouped by
+    // instance) and sums the budgets of consecutive tiles of one instance in
+    // registers: one block reduction per instance run, into the slot of the
+    // run's first tile; the run's other slots hold zeros (the commit sums all
+    // slots of an instance).
+    const int per = (ntile + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int tbeg = (int)blockIdx.x * per, tend = min(ntile, tbeg + per);
+    StepAcc acc;
+    int run_inst = -1, run_slot = 0;
+    auto flush = [&]() {
+        if (run_inst >= 0) {
+            block_partials(acc, B.partials + (long long)run_slot * NPART, &B.ctl[run_inst].pde_key);
+            acc = StepAcc{};
+        }
+    };
+    for (int t = tbeg; t < tend; ++t) {
         const BTile tile = B.tiles[t];
+        if (tile.inst != run_inst) {
+            flush();
+            run_inst = tile.inst;
+            run_slot = t;
+        } else if (tid < NPART) {
+            B.partials[(long long)t * NPART + tid] = 0.;
+        }
         const BInst I = B.inst[tile.inst];
         DevCtl* ctl = B.ctl + tile.inst;
         if (ctl->failed) continue;
@@ -122,7 +145,6 @@ __global__ __launch_bounds__(64 * TX) void k_batch_pde(Par P, BBufs B, int ntile
         }
         __syncthreads();
         // trial of the tile's cells
-        StepAcc acc;
         const int x = tile.x0 + tx, y = tile.y0 + ty;
         if (x < nx && y < ny) {
             const int c1 = (tx + 1) * L::W1 + ty + 1, c2 = (tx + 2) * L::W2 + ty + 2;
@@ -146,9 +168,9 @@ __global__ __launch_bounds__(64 * TX) void k_batch_pde(Par P, BBufs B, int ntile
             finish_cell(P, c, x, o_stored, oo, cc, p7[SC], onv, crv, pnv, left_flux, acc, B.o[nxt] + I.off,
                         B.cr[nxt] + I.off, B.p[nxt] + I.off, I.layer_o);
         }
-        block_partials(acc, B.partials + (long long)t * NPART, &ctl->pde_key);
         __syncthreads();   // LDS is reused by the next tile
     }
+    flush();
 }
 
 // All-or-nothing acceptance of each instance's step (k_commit per instance).
