@@ -197,18 +197,29 @@ def _plain_norm(net, J):
                  gain=[0.05, 0.05, 0.0], jump=J)
 
 
-@pytest.mark.parametrize("n_eta", [0, 8], ids=["static", "evolving"])
-def test_rejected_jumps_equal_plain_steps(nicr, n_eta):
+@pytest.mark.parametrize("n_eta,settle,fallback", [(0, 0, 0), (8, 0, 0), (8, 2, 7)],
+                         ids=["static", "evolving", "evolving-settle2-fallback7"])
+def test_rejected_jumps_equal_plain_steps(nicr, n_eta, settle, fallback):
     """With a residual tolerance no proposal meets, every jump is rejected and
-    the model equals plain stepping bitwise (fields, eta, mask and budgets)."""
+    the model equals plain stepping bitwise (fields, eta, mask and budgets),
+    whatever plain steps the loop inserts (settle and fallback steps)."""
     shape = (16, 16, 24)
     a = _closed_model(nicr, shape, 4, n_eta)
     b = _closed_model(nicr, shape, 4, n_eta)
     net = _sur().UNet(shape, seed=1)
     _plain_norm(net, 20)
-    rep = net.advance(a, 130, jump=20, residual_tol=1e-12)
+    rep = net.advance(a, 130, jump=20, residual_tol=1e-12, settle_steps=settle, fallback_steps=fallback)
     assert rep.status == 0 and rep.steps_done == 130
-    assert rep.jumps_accepted == 0 and rep.jumps_rejected >= 5
+    assert rep.jumps_accepted == 0
+    # attempts: the first after one plain step, then one per (fallback + settle)
+    # steps while at least jump + 1 + settle steps are left
+    fb = fallback or 20
+    left, attempts, pre = 130, 0, 1
+    while left >= 20 + 1 + pre:
+        left -= pre + fb
+        attempts += 1
+        pre = settle
+    assert rep.jumps_rejected == attempts
     assert rep.plain_steps == 130
     rb = b.step(130)
     assert rb.steps_done == 130
