@@ -82,19 +82,15 @@ __global__ __launch_bounds__(64 * TX) void k_batch_pde(Par P, BBufs B, int ntile
             acc = StepAcc{};
         }
     };
-    for (int t = tbeg; t < tend; ++t) {
+    // the next tile's o, cr (halo 2) and phi, gb (halo 1) are loaded into
+    // registers while the current tile computes
+    constexpr int R2 = (L::P2 + NTH - 1) / NTH, R1 = (L::P1 + NTH - 1) / NTH;
+    double st_o[R2], st_c[R2], st_p[R1], st_g[R1];
+    auto fetch = [&](int t) {
         const BTile tile = B.tiles[t];
-        if (tile.inst != run_inst) {
-            flush();
-            run_inst = tile.inst;
-            run_slot = t;
-        } else if (tid < NPART) {
-            B.partials[(long long)t * NPART + tid] = 0.;
-        }
         const BInst I = B.inst[tile.inst];
-        DevCtl* ctl = B.ctl + tile.inst;
-        if (ctl->failed) continue;
-        const int cur = ctl->cur, nxt = cur ^ 1;
+        const DevCtl* ctl = B.ctl + tile.inst;
+        const int cur = ctl->cur;
         const double* o = B.o[cur] + I.off;
         const double* cr = B.cr[cur] + I.off;
         const double* p = B.p[cur] + I.off;
@@ -105,20 +101,67 @@ __global__ __launch_bounds__(64 * TX) void k_batch_pde(Par P, BBufs B, int ntile
         auto wrapy = [&](int v) { return ((v % ny) + ny) % ny; };
         // o, cr over rows x0-2 .. x0+TX+1 (clamped: the X ghosts are copies),
         // columns y0-2 .. y0+65 (periodic); phi, gb over the one-cell halo
-        for (int q = tid; q < L::P2; q += NTH) {
-            const int r = q / L::W2, cy = q - r * L::W2;
-            const int xs = clampx(tile.x0 - 2 + r);
-            const long long g = (long long)xs * ny + wrapy(tile.y0 - 2 + cy);
-            so[q] = (enf && xs == 0) ? I.layer_o : o[g];
-            sc[q] = cr[g];
+#pragma unroll
+        for (int k = 0; k < R2; ++k) {
+            const int q = tid + k * NTH;
+            if (q < L::P2) {
+                const int r = q / L::W2, cy = q - r * L::W2;
+                const int xs = clampx(tile.x0 - 2 + r);
+                const long long g = (long long)xs * ny + wrapy(tile.y0 - 2 + cy);
+                st_o[k] = (enf && xs == 0) ? I.layer_o : o[g];
+                st_c[k] = cr[g];
+            }
         }
-        for (int q = tid; q < L::P1; q += NTH) {
-            const int r = q / L::W1, cy = q - r * L::W1;
-            const long long g = (long long)clampx(tile.x0 - 1 + r) * ny + wrapy(tile.y0 - 1 + cy);
-            sp[q] = p[g];
-            sg[q] = gb[g];
+#pragma unroll
+        for (int k = 0; k < R1; ++k) {
+            const int q = tid + k * NTH;
+            if (q < L::P1) {
+                const int r = q / L::W1, cy = q - r * L::W1;
+                const long long g = (long long)clampx(tile.x0 - 1 + r) * ny + wrapy(tile.y0 - 1 + cy);
+                st_p[k] = p[g];
+                st_g[k] = gb[g];
+            }
+        }
+    };
+    if (tbeg < tend) fetch(tbeg);
+    for (int t = tbeg; t < tend; ++t) {
+        const BTile tile = B.tiles[t];
+        if (tile.inst != run_inst) {
+            flush();
+            run_inst = tile.inst;
+            run_slot = t;
+        } else if (tid < NPART) {
+            B.partials[(long long)t * NPART + tid] = 0.;
+        }
+        // this tile's staged values into LDS, then the next tile's loads
+#pragma unroll
+        for (int k = 0; k < R2; ++k) {
+            const int q = tid + k * NTH;
+            if (q < L::P2) {
+                so[q] = st_o[k];
+                sc[q] = st_c[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R1; ++k) {
+            const int q = tid + k * NTH;
+            if (q < L::P1) {
+                sp[q] = st_p[k];
+                sg[q] = st_g[k];
+            }
         }
         __syncthreads();
+        if (t + 1 < tend) fetch(t + 1);
+        const BInst I = B.inst[tile.inst];
+        DevCtl* ctl = B.ctl + tile.inst;
+        if (ctl->failed) {
+            __syncthreads();   // the LDS tile is rewritten by the next iteration
+            continue;
+        }
+        const int cur = ctl->cur, nxt = cur ^ 1;
+        const double* o = B.o[cur] + I.off;
+        const int nx = I.nx, ny = I.ny;
+        auto clampx = [&](int v) { return v < 0 ? 0 : (v >= nx ? nx - 1 : v); };
         // mu and M over the tile and its halo. A halo row outside the mesh is
         // the ghost copy of its clamped row, mu included (2D:427), so it is
         // evaluated at the clamped row.
