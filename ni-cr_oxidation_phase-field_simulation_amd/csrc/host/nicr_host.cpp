@@ -18,6 +18,7 @@
 #include <locale>
 #include <random>
 #include <sstream>
+#include <thread>
 
 namespace fs = std::filesystem;
 
@@ -725,6 +726,61 @@ void HostModel::metadata(const std::string& backend) const {
 // ---------------------------------------------------------------------------
 // visualisation writers
 // ---------------------------------------------------------------------------
+// Output work on the host cores: fn(begin, end) over contiguous chunks of
+// [0, n) on up to 16 threads (one chunk, no thread, for small n). The writers
+// keep their byte order: chunks produce independent pieces that are written
+// in sequence.
+static int writer_threads() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return static_cast<int>(std::max(1u, std::min(16u, hw)));
+}
+template <class Fn>
+static void parallel_chunks(std::size_t n, std::size_t min_chunk, Fn fn) {
+    const std::size_t nt = std::min<std::size_t>(writer_threads(), std::max<std::size_t>(1, n / min_chunk));
+    if (nt <= 1) {
+        fn(std::size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    std::vector<std::exception_ptr> err(nt);
+    for (std::size_t t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            try {
+                fn(n * t / nt, n * (t + 1) / nt);
+            } catch (...) {
+                err[t] = std::current_exception();
+            }
+        });
+    for (auto& th : pool) th.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+}
+
+// ASCII rows of one VTK array, formatted by worker threads with the stream's
+// own locale, precision and flags, then written in order: rows(r, out) writes
+// row r (of nrows) to out.
+template <class RowFn>
+static void vtk_rows(std::ostream& f, std::size_t nrows, RowFn rows) {
+    const std::size_t batch = std::max<std::size_t>(1, 4096);
+    for (std::size_t r0 = 0; r0 < nrows; r0 += batch * writer_threads()) {
+        const std::size_t r1 = std::min(nrows, r0 + batch * writer_threads());
+        const std::size_t nt = std::max<std::size_t>(1, (r1 - r0 + batch - 1) / batch);
+        std::vector<std::string> piece(nt);
+        parallel_chunks(nt, 1, [&](std::size_t a, std::size_t b) {
+            for (std::size_t t = a; t < b; ++t) {
+                std::ostringstream out;
+                out.imbue(f.getloc());
+                out.precision(f.precision());
+                out.flags(f.flags());
+                const std::size_t lo = r0 + t * batch, hi = std::min(r1, lo + batch);
+                for (std::size_t r = lo; r < hi; ++r) rows(r, out);
+                piece[t] = out.str();
+            }
+        });
+        for (const auto& sp : piece) f.write(sp.data(), static_cast<std::streamsize>(sp.size()));
+    }
+}
+
 void HostModel::write_vtk(const std::string& path) const {
     const nicr_config& c = hc.c;
     std::ofstream f(path);
@@ -738,10 +794,11 @@ void HostModel::write_vtk(const std::string& path) const {
           << c.dx << " " << c.dy << " 1\nPOINT_DATA " << nx * ny << "\n";
         auto write = [&](const std::string& name, auto fn) {
             f << "SCALARS " << name << " double 1\nLOOKUP_TABLE default\n";
-            for (int j = 1; j <= ny; ++j) {
-                for (int i = 1; i <= nx; ++i) f << fn(at(i, j)) << " ";
-                f << "\n";
-            }
+            vtk_rows(f, static_cast<std::size_t>(ny), [&](std::size_t r, std::ostream& out) {
+                const int j = static_cast<int>(r) + 1;
+                for (int i = 1; i <= nx; ++i) out << fn(at(i, j)) << " ";
+                out << "\n";
+            });
         };
         write("phi", [&](std::size_t q) { return p[q]; });
         write("conc_O", [&](std::size_t q) { return o[q]; });
@@ -772,11 +829,11 @@ void HostModel::write_vtk(const std::string& path) const {
           << "\nPOINT_DATA " << nx * ny * nz << "\n";
         auto write = [&](const char* name, auto fn) {
             f << "SCALARS " << name << " double 1\nLOOKUP_TABLE default\n";
-            for (int k = 1; k <= nz; ++k)
-                for (int j = 1; j <= ny; ++j) {
-                    for (int i = 1; i <= nx; ++i) f << fn(at(i, j, k)) << " ";
-                    f << "\n";
-                }
+            vtk_rows(f, static_cast<std::size_t>(nz) * ny, [&](std::size_t r, std::ostream& out) {
+                const int k = static_cast<int>(r / ny) + 1, j = static_cast<int>(r % ny) + 1;
+                for (int i = 1; i <= nx; ++i) out << fn(at(i, j, k)) << " ";
+                out << "\n";
+            });
         };
         write("phi", [&](std::size_t q) { return p[q]; });
         write("conc_O", [&](std::size_t q) { return o[q]; });
@@ -817,42 +874,40 @@ class VtiWriter {
     void blocks(Fn value) {
         const std::uint64_t n = std::uint64_t(m.nx) * m.ny * m.nz, total = n * sizeof(T);
         const std::uint64_t nb = (total + kBlock - 1) / kBlock;
-        const auto header_at = f.tellp();
-        std::vector<std::uint64_t> sizes(static_cast<std::size_t>(nb), 0);
-        put_u64(nb);
-        put_u64(kBlock);
-        put_u64(total % kBlock);
-        for (std::uint64_t q = 0; q < nb; ++q) put_u64(0);
-        std::vector<T> buf(static_cast<std::size_t>(kBlock / sizeof(T)));
-        std::vector<unsigned char> packed(static_cast<std::size_t>(compressBound(kBlock)));
-        std::size_t used = 0, block = 0;
-        auto flush = [&] {
-            if (!used) return;
-            uLongf dest = static_cast<uLongf>(packed.size());
-            const int rc = compress2(packed.data(), &dest, reinterpret_cast<const Bytef*>(buf.data()),
-                                     static_cast<uLong>(used * sizeof(T)), level);
-            if (rc != Z_OK) throw std::runtime_error("zlib compression failed: " + std::to_string(rc));
-            sizes.at(block++) = dest;
-            f.write(reinterpret_cast<const char*>(packed.data()), static_cast<std::streamsize>(dest));
-            check();
-            used = 0;
-        };
-        for (int k = 1; k <= m.nz; ++k)
-            for (int j = 1; j <= m.ny; ++j)
+        // the array in X-fastest order, gathered by worker threads over (z, y) rows
+        std::vector<T> all(static_cast<std::size_t>(n));
+        const std::size_t nrow = static_cast<std::size_t>(m.nz) * m.ny;
+        parallel_chunks(nrow, 64, [&](std::size_t a, std::size_t b) {
+            for (std::size_t r = a; r < b; ++r) {
+                const int k = static_cast<int>(r / m.ny) + 1, j = static_cast<int>(r % m.ny) + 1;
+                T* row = all.data() + r * m.nx;
                 for (int i = 1; i <= m.nx; ++i) {
                     const double v = value(m.at(i, j, k));
                     if (std::isfinite(v) && std::abs(v) > std::numeric_limits<T>::max())
                         throw std::runtime_error(std::string("VTI ") + kNames[next] +
                                                  " exceeds output type range; use --output-precision float64.");
-                    buf[used++] = static_cast<T>(v);
-                    if (used == buf.size()) flush();
+                    row[i - 1] = static_cast<T>(v);
                 }
-        flush();
-        if (block != sizes.size()) throw std::runtime_error("Internal VTI block-count mismatch.");
-        const auto end = f.tellp();
-        f.seekp(header_at + std::streamoff(24));
-        for (auto s : sizes) put_u64(s);
-        f.seekp(end);
+            }
+        });
+        // independent zlib streams of kBlock bytes, compressed by worker threads
+        std::vector<std::vector<unsigned char>> packed(static_cast<std::size_t>(nb));
+        parallel_chunks(static_cast<std::size_t>(nb), 4, [&](std::size_t a, std::size_t b) {
+            for (std::size_t q = a; q < b; ++q) {
+                const std::uint64_t lo = q * kBlock, len = std::min<std::uint64_t>(kBlock, total - lo);
+                packed[q].resize(static_cast<std::size_t>(compressBound(static_cast<uLong>(len))));
+                uLongf dest = static_cast<uLongf>(packed[q].size());
+                const int rc = compress2(packed[q].data(), &dest,
+                                         reinterpret_cast<const Bytef*>(all.data()) + lo, static_cast<uLong>(len), level);
+                if (rc != Z_OK) throw std::runtime_error("zlib compression failed: " + std::to_string(rc));
+                packed[q].resize(static_cast<std::size_t>(dest));
+            }
+        });
+        put_u64(nb);
+        put_u64(kBlock);
+        put_u64(total % kBlock);
+        for (const auto& b : packed) put_u64(b.size());
+        for (const auto& b : packed) f.write(reinterpret_cast<const char*>(b.data()), static_cast<std::streamsize>(b.size()));
         check();
     }
 
@@ -1072,6 +1127,25 @@ int nicr_host_write(void* h, int what, const char* path, int step, const char* o
                 else if (k == "ck") m.hc.ck = v;
                 else if (k == "bounds") m.hc.bounds = v;
             }
+        }
+        // the writers read the refreshed arrays (mu, M, eta sums) that the caller
+        // uploads after a refresh; a missing one is an error, not a stray read
+        auto need = [&](const std::vector<double>& v, const char* name) {
+            if (v.size() != m.ncell)
+                throw std::runtime_error(std::string("nicr_host_write: ") + name +
+                                         " is not set (upload the refreshed field first)");
+        };
+        if (what == 3 || what == 4) {
+            need(m.muO, "mu_O");
+            need(m.muCr, "mu_Cr");
+        }
+        if (what == 3 && m.dim == 2) {
+            need(m.mo, "M_O");
+            need(m.mc, "M_Cr");
+        }
+        if ((what == 2 || (what == 3 && m.dim == 2)) && m.has_eta()) {
+            need(m.eta_s2, "eta_s2");
+            if (m.dim == 2) need(m.eta_pairs, "eta_pairs");
         }
         if (what == 0) {
             m.hc.out = path;

@@ -71,3 +71,15 @@ def test_3d_writers_byte_identical(nicr, oracle, tmp_path, case):
         hs.write("vti", str(mine / f"f_{prec}.vti"), options=f"precision={prec};level=1")
         assert w(ref._h.h, 4, str(theirs / f"f_{prec}.vti").encode(), 0, prec.encode()) == 0
         assert _bytes(mine / f"f_{prec}.vti") == _bytes(theirs / f"f_{prec}.vti"), prec
+
+
+def test_writers_refuse_unset_refreshed_fields(tmp_path):
+    """The VTK/VTI writers read mu (and M, eta sums) uploaded after a refresh; a
+    host state without them is an error, not a stray read."""
+    host = importlib.import_module("ni-cr_oxidation_phase-field_simulation_amd.host")
+    cfg, opt = host.parse_args(3, ["--nx", "12", "--ny", "14", "--nz", "14", "--bounds", "legacy"])
+    hs = host.HostState(cfg, opt["bc"], opt["nucleation"])
+    for what in ("vti", "vtk"):
+        with pytest.raises(RuntimeError, match="mu_O is not set"):
+            hs.write(what, str(tmp_path / f"f.{what}"))
+    hs.close()
