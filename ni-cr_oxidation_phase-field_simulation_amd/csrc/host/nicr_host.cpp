@@ -18,6 +18,7 @@
 #include <locale>
 #include <random>
 #include <sstream>
+#include <mutex>
 #include <thread>
 
 namespace fs = std::filesystem;
@@ -238,6 +239,36 @@ HostModel::HostModel(const HostConfig& cfg) : hc(cfg) {
     if (has_eta()) initialize_eta();
 }
 
+// Host work on up to 16 threads: fn(begin, end) over contiguous chunks of
+// [0, n) (one chunk, no thread, for small n). Used where every item is
+// independent (cells of the initial state, output blocks and rows), so the
+// results do not depend on the split.
+static int writer_threads() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return static_cast<int>(std::max(1u, std::min(16u, hw)));
+}
+template <class Fn>
+static void parallel_chunks(std::size_t n, std::size_t min_chunk, Fn fn) {
+    const std::size_t nt = std::min<std::size_t>(writer_threads(), std::max<std::size_t>(1, n / min_chunk));
+    if (nt <= 1) {
+        fn(std::size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    std::vector<std::exception_ptr> err(nt);
+    for (std::size_t t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            try {
+                fn(n * t / nt, n * (t + 1) / nt);
+            } catch (...) {
+                err[t] = std::current_exception();
+            }
+        });
+    for (auto& th : pool) th.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+}
+
 double HostModel::competitor_distance(std::size_t g, int owner, double x, double y, double z, double oy, double oz,
                                       double rbest) const {
     double db = std::numeric_limits<double>::infinity();
@@ -323,7 +354,11 @@ void HostModel::voronoi() {
         }
     }
     areas.assign(ng, 0);
-    for (int i = 1; i <= nx; ++i)
+    std::mutex areas_lock;
+    // X planes on host threads; each cell depends only on the seeds
+    parallel_chunks(static_cast<std::size_t>(nx), 1, [&](std::size_t ia, std::size_t ib) {
+    std::vector<long long> my_areas(ng, 0);
+    for (int i = static_cast<int>(ia) + 1; i <= static_cast<int>(ib); ++i)
         for (int j = 1; j <= ny; ++j)
             for (int k = 1; k <= nz; ++k) {
                 const std::size_t q = at(i, j, k);
@@ -344,7 +379,7 @@ void HostModel::voronoi() {
                     }
                 }
                 grain[q] = owner + 1;
-                ++areas[owner];
+                ++my_areas[owner];
                 double db = std::numeric_limits<double>::infinity();
                 if (dim == 2) {
                     for (int g = 0; g < ng; ++g)
@@ -361,6 +396,9 @@ void HostModel::voronoi() {
                 }
                 gb[q] = ng == 1 ? 0. : std::exp(-4. * std::log(2.) * db * db / (c.gb_fwhm_grid * c.gb_fwhm_grid));
             }
+    std::lock_guard<std::mutex> guard(areas_lock);
+    for (int g = 0; g < ng; ++g) areas[g] += my_areas[g];
+    });
 }
 
 void HostModel::initialize_eta() {
@@ -370,8 +408,10 @@ void HostModel::initialize_eta() {
                                           : "Too many eta fields; check --max-eta-fields and memory.");
     eta.assign(seeds.size(), std::vector<double>(ncell, 0.));
     const double ell_grid = eta_ell(c) / c.dx;
+    // X planes on host threads; each cell depends only on the seeds and its owner
+    parallel_chunks(static_cast<std::size_t>(nx), 1, [&](std::size_t ia, std::size_t ib) {
     std::vector<double> weights(seeds.size());
-    for (int i = 1; i <= nx; ++i)
+    for (int i = static_cast<int>(ia) + 1; i <= static_cast<int>(ib); ++i)
         for (int j = 1; j <= ny; ++j)
             for (int k = 1; k <= nz; ++k) {
                 const std::size_t q = at(i, j, k);
@@ -406,6 +446,7 @@ void HostModel::initialize_eta() {
                 const double metal = 1. - h(p[q]);
                 for (std::size_t g = 0; g < eta.size(); ++g) eta[g][q] = metal * weights[g] / norm;
             }
+    });
 }
 
 void HostModel::initialize_center_oxide() {
@@ -726,36 +767,6 @@ void HostModel::metadata(const std::string& backend) const {
 // ---------------------------------------------------------------------------
 // visualisation writers
 // ---------------------------------------------------------------------------
-// Output work on the host cores: fn(begin, end) over contiguous chunks of
-// [0, n) on up to 16 threads (one chunk, no thread, for small n). The writers
-// keep their byte order: chunks produce independent pieces that are written
-// in sequence.
-static int writer_threads() {
-    const unsigned hw = std::thread::hardware_concurrency();
-    return static_cast<int>(std::max(1u, std::min(16u, hw)));
-}
-template <class Fn>
-static void parallel_chunks(std::size_t n, std::size_t min_chunk, Fn fn) {
-    const std::size_t nt = std::min<std::size_t>(writer_threads(), std::max<std::size_t>(1, n / min_chunk));
-    if (nt <= 1) {
-        fn(std::size_t(0), n);
-        return;
-    }
-    std::vector<std::thread> pool;
-    std::vector<std::exception_ptr> err(nt);
-    for (std::size_t t = 0; t < nt; ++t)
-        pool.emplace_back([&, t] {
-            try {
-                fn(n * t / nt, n * (t + 1) / nt);
-            } catch (...) {
-                err[t] = std::current_exception();
-            }
-        });
-    for (auto& th : pool) th.join();
-    for (auto& e : err)
-        if (e) std::rethrow_exception(e);
-}
-
 // ASCII rows of one VTK array, formatted by worker threads with the stream's
 // own locale, precision and flags, then written in order: rows(r, out) writes
 // row r (of nrows) to out.
