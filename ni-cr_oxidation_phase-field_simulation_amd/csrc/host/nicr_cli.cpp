@@ -10,6 +10,9 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <filesystem>
 #include <fstream>
 #include <iomanip>
@@ -43,6 +46,26 @@ std::string backend_name() {
         return std::string("hip_") + prop.gcnArchName + "_single_gpu";
     return "hip_single_gpu";
 }
+
+// Wall time per phase of the run, summed over the run and printed to stderr at
+// exit when the environment sets NICR_PHASE_TIMES; the outputs do not change.
+struct PhaseTimes {
+    bool on = std::getenv("NICR_PHASE_TIMES") != nullptr;
+    std::vector<std::pair<std::string, double>> acc;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void mark(const char* name) {
+        if (!on) return;
+        const auto now = std::chrono::steady_clock::now();
+        const double d = std::chrono::duration<double>(now - t).count();
+        t = now;
+        for (auto& a : acc)
+            if (a.first == name) { a.second += d; return; }
+        acc.emplace_back(name, d);
+    }
+    ~PhaseTimes() {
+        for (const auto& a : acc) std::fprintf(stderr, "phase %-10s %9.3f s\n", a.first.c_str(), a.second);
+    }
+};
 
 long double join(const double v[2]) { return (long double)v[0] + (long double)v[1]; }
 
@@ -251,6 +274,7 @@ void self_test(int dim) {
 
 int run(int argc, char** argv) {
     const int dim = NICR_CLI_DIM;
+    PhaseTimes pt;
     try {
         HostConfig hc = parse_args(dim, argc, argv, std::cout);
         if (hc.self_test) {
@@ -261,14 +285,18 @@ int run(int argc, char** argv) {
         if (fs::exists(hc.out) && !fs::is_empty(hc.out)) throw std::runtime_error("Output directory is not empty; choose a new --out.");
         fs::create_directories(hc.out);
         HostModel m(hc);
+        pt.mark("init");
         const nicr_config& c = hc.c;
         nicr_config dc = c;
         dc.n_eta = int(m.eta.size());
         Device dev;
         check(nicr_create(&dc, &dev.ctx), "nicr_create");
+        pt.mark("create");
         upload(m, dev.ctx);
+        pt.mark("upload");
         const std::string backend = backend_name();
         m.metadata(backend);
+        pt.mark("metadata");
         std::ofstream log, grain_diag, grain_history;
         log.open(hc.out + "/diagnostics.csv");
         if (!log) throw std::runtime_error("Cannot open diagnostics.csv");
@@ -287,10 +315,15 @@ int run(int argc, char** argv) {
                           << (dim == 2 ? "weighted_metal_area_cells,weighted_metal_area_m2\n"
                                        : "weighted_metal_volume_cells,weighted_metal_volume_m3\n");
         }
+        pt.mark("metadata");
         download(m, dev.ctx);
+        pt.mark("download");
         m.log_row(log, 0);
+        pt.mark("log");
         m.grain_log(grain_diag, grain_history, 0);
+        pt.mark("grain_log");
         m.write_visualization(hc.out + "/fields_0");
+        pt.mark("write");
 
         std::cout << "EXECUTION BACKEND: " << backend << " (HIP device step; no MPI).\n";
         std::cout << "MPI processes: 1 (X slabs)\n";
@@ -349,7 +382,9 @@ int run(int argc, char** argv) {
                 const double time = n * c.dt;
                 if (hc.nucleation == "ksp" && time + 1.e-12 * c.nucleation_period >= next_nucleation) {
                     int cells = 0;
+                    pt.mark("host");
                     check(nicr_nucleate(dev.ctx, &cells), "nucleate");
+                    pt.mark("nucleate");
                     next_nucleation += c.nucleation_period;
                 }
                 // steps n..e-1 contain no event: no scan due, no output, no progress line
@@ -364,7 +399,9 @@ int run(int argc, char** argv) {
                 }
                 const int batch = e - n;
                 nicr_step_report r{};
+                pt.mark("host");
                 check(nicr_step(dev.ctx, batch, &r), "step");
+                pt.mark("step");
                 if (r.status != NICR_STEP_OK) {
                     last_completed = n - 1 + r.steps_done;
                     throw std::runtime_error(step_message(r, dim));
@@ -376,9 +413,13 @@ int run(int argc, char** argv) {
                     std::cout << "step " << done << "/" << c.steps << " (" << 100.0 * done / c.steps << "%), time=" << done * c.dt
                               << " s" << std::endl;
                 if (done % c.output_every == 0 || done == c.steps) {
+                    pt.mark("host");
                     download(m, dev.ctx);
+                    pt.mark("download");
                     m.log_row(log, done);
+                    pt.mark("log");
                     m.grain_log(grain_diag, grain_history, done);
+                    pt.mark("grain_log");
                     log.flush();
                     if (!log) throw std::runtime_error("Diagnostics write failed.");
                     if (m.has_eta()) {
@@ -387,6 +428,7 @@ int run(int argc, char** argv) {
                         if (!grain_diag || !grain_history) throw std::runtime_error("Grain diagnostics write failed.");
                     }
                     m.write_visualization(hc.out + "/fields_" + std::to_string(done));
+                    pt.mark("write");
                 }
             }
         } catch (const std::exception& e) {

@@ -2,6 +2,7 @@
 // See nicr_host.hpp. Output formats (key order, precision, array order) follow
 // the reference byte for byte; they are interfaces of the programs.
 #include "nicr_host.hpp"
+#include "../host_threads.hpp"
 #include "../../../include/nicr_host.h"
 
 #include <zlib.h>
@@ -19,7 +20,6 @@
 #include <random>
 #include <sstream>
 #include <mutex>
-#include <thread>
 
 namespace fs = std::filesystem;
 
@@ -239,35 +239,7 @@ HostModel::HostModel(const HostConfig& cfg) : hc(cfg) {
     if (has_eta()) initialize_eta();
 }
 
-// Host work on up to 16 threads: fn(begin, end) over contiguous chunks of
-// [0, n) (one chunk, no thread, for small n). Used where every item is
-// independent (cells of the initial state, output blocks and rows), so the
-// results do not depend on the split.
-static int writer_threads() {
-    const unsigned hw = std::thread::hardware_concurrency();
-    return static_cast<int>(std::max(1u, std::min(16u, hw)));
-}
-template <class Fn>
-static void parallel_chunks(std::size_t n, std::size_t min_chunk, Fn fn) {
-    const std::size_t nt = std::min<std::size_t>(writer_threads(), std::max<std::size_t>(1, n / min_chunk));
-    if (nt <= 1) {
-        fn(std::size_t(0), n);
-        return;
-    }
-    std::vector<std::thread> pool;
-    std::vector<std::exception_ptr> err(nt);
-    for (std::size_t t = 0; t < nt; ++t)
-        pool.emplace_back([&, t] {
-            try {
-                fn(n * t / nt, n * (t + 1) / nt);
-            } catch (...) {
-                err[t] = std::current_exception();
-            }
-        });
-    for (auto& th : pool) th.join();
-    for (auto& e : err)
-        if (e) std::rethrow_exception(e);
-}
+using nicr_threads::parallel_chunks;
 
 double HostModel::competitor_distance(std::size_t g, int owner, double x, double y, double z, double oy, double oz,
                                       double rbest) const {
@@ -507,58 +479,96 @@ static double local_energy(double S, double P, double phi, const nicr_config& c)
 long double HostModel::grain_energy() const {
     if (!has_eta()) return 0.;
     const nicr_config& c = hc.c;
-    long double local = 0.;
-    for (int i = 1; i <= nx; ++i)
-        for (int j = 1; j <= ny; ++j)
-            for (int k = 1; k <= nz; ++k) {
-                const std::size_t q = at(i, j, k);
-                double density;
-                if (dim == 2) {
-                    density = local_energy(eta_s2[q], eta_pairs[q], p[q], c);
-                } else {
-                    double S = 0., pairs = 0.;
-                    for (const auto& e : eta) { const double v = e[q] * e[q]; pairs += S * v; S += v; }
-                    density = local_energy(eta_s2[q], pairs, p[q], c);
-                }
-                const int jp = j == ny ? 1 : j + 1;
-                for (const auto& e : eta) {
-                    if (i < nx) density += .5 * c.eta_K * std::pow((e[at(i + 1, j, k)] - e[q]) / c.dx, 2);
-                    density += .5 * c.eta_K * std::pow((e[at(i, jp, k)] - e[q]) / c.dy, 2);
-                    if (dim == 3) {
-                        const int kp = k == nz ? 1 : k + 1;
-                        density += .5 * c.eta_K * std::pow((e[at(i, j, kp)] - e[q]) / c.dz, 2);
+    // each cell's term (a double) on the host threads, then the long double
+    // sum in the serial loop order, so the total is bitwise that of one thread
+    std::vector<double> term(ncell);
+    parallel_chunks(std::size_t(nx), 1, [&](std::size_t i0, std::size_t i1) {
+        for (int i = int(i0) + 1; i <= int(i1); ++i)
+            for (int j = 1; j <= ny; ++j)
+                for (int k = 1; k <= nz; ++k) {
+                    const std::size_t q = at(i, j, k);
+                    double density;
+                    if (dim == 2) {
+                        density = local_energy(eta_s2[q], eta_pairs[q], p[q], c);
+                    } else {
+                        double S = 0., pairs = 0.;
+                        for (const auto& e : eta) { const double v = e[q] * e[q]; pairs += S * v; S += v; }
+                        density = local_energy(eta_s2[q], pairs, p[q], c);
                     }
+                    const int jp = j == ny ? 1 : j + 1;
+                    for (const auto& e : eta) {
+                        if (i < nx) density += .5 * c.eta_K * std::pow((e[at(i + 1, j, k)] - e[q]) / c.dx, 2);
+                        density += .5 * c.eta_K * std::pow((e[at(i, jp, k)] - e[q]) / c.dy, 2);
+                        if (dim == 3) {
+                            const int kp = k == nz ? 1 : k + 1;
+                            density += .5 * c.eta_K * std::pow((e[at(i, j, kp)] - e[q]) / c.dz, 2);
+                        }
+                    }
+                    term[q] = dim == 2 ? density * c.dx * c.dy : density * c.dx * c.dy * c.dz;
                 }
-                local += dim == 2 ? density * c.dx * c.dy : density * c.dx * c.dy * c.dz;
-            }
+    });
+    long double local = 0.;
+    for (std::size_t q = 0; q < ncell; ++q) local += term[q];
     return local;
 }
 
 void HostModel::grain_log(std::ostream& f, std::ostream& history, int n) const {
     if (!has_eta()) return;
     const nicr_config& c = hc.c;
-    std::vector<long double> amount(eta.size(), 0.);
-    std::vector<int> cells(eta.size(), 0);
+    const std::size_t ng = eta.size();
+    std::vector<long double> amount(ng, 0.);
+    std::vector<int> cells(ng, 0);
     double lo = 1., hi = 0., slo = std::numeric_limits<double>::infinity(), shi = 0., gbmax = 0.;
-    long double sumS = 0., mask = 0.;
     int unassigned = 0;
-    for (std::size_t q = 0; q < ncell; ++q) {
-        const double S = eta_s2[q], metal = 1. - h(p[q]);
-        slo = std::min(slo, S);
-        shi = std::max(shi, S);
-        sumS += S;
-        mask += metal * gb[q];
-        gbmax = std::max(gbmax, gb[q]);
-        const int id = current_grain(q);
-        if (id) ++cells[std::size_t(id - 1)];
-        else if (p[q] < .5) ++unassigned;
-        for (std::size_t g = 0; g < eta.size(); ++g) {
-            const double v = eta[g][q];
-            lo = std::min(lo, v);
-            hi = std::max(hi, v);
-            if (S > 1.e-24) amount[g] += metal * v * v / S;
+    std::vector<double> metal(ncell);
+    // extrema and counts do not depend on the order: per-thread partials, merged
+    std::mutex merge;
+    parallel_chunks(ncell, 1 << 14, [&](std::size_t q0, std::size_t q1) {
+        double tlo = 1., thi = 0., tslo = std::numeric_limits<double>::infinity(), tshi = 0., tgb = 0.;
+        int tun = 0;
+        std::vector<int> tcells(ng, 0);
+        for (std::size_t q = q0; q < q1; ++q) {
+            const double S = eta_s2[q];
+            metal[q] = 1. - h(p[q]);
+            tslo = std::min(tslo, S);
+            tshi = std::max(tshi, S);
+            tgb = std::max(tgb, gb[q]);
+            const int id = current_grain(q);
+            if (id) ++tcells[std::size_t(id - 1)];
+            else if (p[q] < .5) ++tun;
+            for (std::size_t g = 0; g < ng; ++g) {
+                tlo = std::min(tlo, eta[g][q]);
+                thi = std::max(thi, eta[g][q]);
+            }
         }
+        std::lock_guard<std::mutex> lock(merge);
+        lo = std::min(lo, tlo);
+        hi = std::max(hi, thi);
+        slo = std::min(slo, tslo);
+        shi = std::max(shi, tshi);
+        gbmax = std::max(gbmax, tgb);
+        unassigned += tun;
+        for (std::size_t g = 0; g < ng; ++g) cells[g] += tcells[g];
+    });
+    // long double sums keep the serial cell order: one pass for the scalars,
+    // and each thread owns whole grains for the per-grain amounts
+    long double sumS = 0., mask = 0.;
+    for (std::size_t q = 0; q < ncell; ++q) {
+        sumS += eta_s2[q];
+        mask += metal[q] * gb[q];
     }
+    parallel_chunks(ng, 1, [&](std::size_t g0, std::size_t g1) {
+        std::vector<long double> acc(g1 - g0, 0.);
+        for (std::size_t q = 0; q < ncell; ++q) {
+            const double S = eta_s2[q];
+            if (S > 1.e-24)
+                for (std::size_t g = g0; g < g1; ++g) {
+                    const double v = eta[g][q];
+                    acc[g - g0] += metal[q] * v * v / S;
+                }
+        }
+        std::copy(acc.begin(), acc.end(), amount.begin() + std::ptrdiff_t(g0));
+    });
     const auto alive = std::count_if(cells.begin(), cells.end(), [](int v) { return v > 0; });
     // long double products evaluated left to right, as the reference writes them
     auto per_volume = [&](long double v) { return dim == 2 ? v * c.dx * c.dy : v * c.dx * c.dy * c.dz; };
@@ -773,8 +783,8 @@ void HostModel::metadata(const std::string& backend) const {
 template <class RowFn>
 static void vtk_rows(std::ostream& f, std::size_t nrows, RowFn rows) {
     const std::size_t batch = std::max<std::size_t>(1, 4096);
-    for (std::size_t r0 = 0; r0 < nrows; r0 += batch * writer_threads()) {
-        const std::size_t r1 = std::min(nrows, r0 + batch * writer_threads());
+    for (std::size_t r0 = 0; r0 < nrows; r0 += batch * nicr_threads::host_threads()) {
+        const std::size_t r1 = std::min(nrows, r0 + batch * nicr_threads::host_threads());
         const std::size_t nt = std::max<std::size_t>(1, (r1 - r0 + batch - 1) / batch);
         std::vector<std::string> piece(nt);
         parallel_chunks(nt, 1, [&](std::size_t a, std::size_t b) {

@@ -7,6 +7,7 @@
 #include "nicr_kernels.hpp"
 #include "nicr_step.hip"
 #include "nicr_nucleate.hip"
+#include "host_threads.hpp"
 
 #include <cmath>
 #include <cstdio>
@@ -65,6 +66,7 @@ struct Ctx {
     double* stat_part = nullptr;
     double* out4[4] = {nullptr, nullptr, nullptr, nullptr};  // refresh outputs muO, muCr, mo, mc
     double* dev_obuf = nullptr;  // device copy of {o0, o1, cr0, cr1, p0, p1}
+    float* stage = nullptr;      // pinned host copy of one binary32 field (first conversion on)
     int* nuc_tiles = nullptr;    // nucleation scratch (allocated on first use)
     int* nuc_planes = nullptr;
     int* nuc_list = nullptr;
@@ -707,6 +709,7 @@ int nicr_destroy(void* h) {
     hipStreamSynchronize(c->stream);
     for (void* p : c->allocs) hipFree(p);
     for (hipEvent_t e : c->evpool) hipEventDestroy(e);
+    if (c->stage) hipHostFree(c->stage);
     hipStreamDestroy(c->stream);
     delete c;
     return NICR_OK;
@@ -719,6 +722,14 @@ int nicr_device_bytes(void* h, size_t* bytes) {
 }
 
 // Host layout == device layout (interior cells, reference storage order).
+// Binary32 fields pass through one pinned host buffer of ncell floats, kept
+// until nicr_destroy; every transfer waits for its copy, so calls on one
+// context never share it in flight.
+static int pinned_stage(Ctx* c) {
+    if (!c->stage) HIPCHK(hipHostMalloc((void**)&c->stage, (size_t)c->ncell * 4, hipHostMallocDefault));
+    return NICR_OK;
+}
+
 int nicr_upload_field(void* h, int field, const double* src) {
     Ctx* c = static_cast<Ctx*>(h);
     if (!c || !src) return fail(NICR_E_INVALID, "nicr_upload_field: null argument");
@@ -745,9 +756,14 @@ int nicr_upload_field(void* h, int field, const double* src) {
     }
     c->refresh_valid = false;
     if (convert) {
-        std::vector<float> tmp((size_t)c->ncell);
-        for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)src[i];
-        HIPCHK(hipMemcpyAsync(dst, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice, c->stream));
+        // rounded to binary32 on host threads into pinned memory, one DMA copy
+        int rc = pinned_stage(c);
+        if (rc) return rc;
+        float* st = c->stage;
+        nicr_threads::parallel_chunks((size_t)c->ncell, size_t(1) << 18, [&](size_t a, size_t b) {
+            for (size_t i = a; i < b; ++i) st[i] = (float)src[i];
+        });
+        HIPCHK(hipMemcpyAsync(dst, st, (size_t)c->ncell * 4, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     } else {
         HIPCHK(hipMemcpyAsync(dst, src, n8, hipMemcpyHostToDevice, c->stream));
@@ -792,10 +808,14 @@ int nicr_download_field(void* h, int field, double* dstp) {
         c->mask_dirty = false;
     }
     if (convert) {
-        std::vector<float> tmp((size_t)c->ncell);
-        HIPCHK(hipMemcpyAsync(tmp.data(), src, tmp.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        int rc = pinned_stage(c);
+        if (rc) return rc;
+        const float* st = c->stage;
+        HIPCHK(hipMemcpyAsync(c->stage, src, (size_t)c->ncell * 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        for (size_t i = 0; i < tmp.size(); ++i) dstp[i] = (double)tmp[i];
+        nicr_threads::parallel_chunks((size_t)c->ncell, size_t(1) << 18, [&](size_t a, size_t b) {
+            for (size_t i = a; i < b; ++i) dstp[i] = (double)st[i];
+        });
     } else {
         HIPCHK(hipMemcpyAsync(dstp, src, (size_t)c->ncell * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));

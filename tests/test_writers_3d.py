@@ -20,6 +20,10 @@ CASES = [
      "--bounds", "legacy", "--nucleation", "off"],
     ["--nx", "17", "--ny", "11", "--nz", "9", "--grains", "off", "--bc", "closed", "--ck", "off",
      "--bounds", "legacy", "--nucleation", "center", "--radius-grid", "3"],
+    # 36,000 cells and 40 grains: the grain rows are reduced over several host
+    # threads (cell chunks of 16,384 and whole grains per thread)
+    ["--nx", "40", "--ny", "30", "--nz", "30", "--grain-diameter-grid", "12", "--grain-evolution", "on",
+     "--bc", "closed", "--bounds", "legacy", "--nucleation", "off"],
 ]
 
 
