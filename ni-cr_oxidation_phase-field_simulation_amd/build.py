@@ -24,7 +24,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unused-value", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra", "-pthread"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fno-semantic-interposition", "-ffp-contract=off", "-Wall", "-Wextra", "-pthread"]
 
 
 def _run(cmd, quiet):

@@ -13,7 +13,10 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <exception>
 #include <filesystem>
+#include <functional>
+#include <thread>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -56,14 +59,61 @@ struct PhaseTimes {
     void mark(const char* name) {
         if (!on) return;
         const auto now = std::chrono::steady_clock::now();
-        const double d = std::chrono::duration<double>(now - t).count();
+        add(name, std::chrono::duration<double>(now - t).count());
         t = now;
+    }
+    void add(const std::string& name, double d) {
+        if (!on) return;
         for (auto& a : acc)
             if (a.first == name) { a.second += d; return; }
         acc.emplace_back(name, d);
     }
     ~PhaseTimes() {
         for (const auto& a : acc) std::fprintf(stderr, "phase %-10s %9.3f s\n", a.first.c_str(), a.second);
+    }
+};
+
+// One output snapshot (diagnostics row, grain rows, visualisation file) written
+// on a host thread while the device runs the next steps. The host arrays of
+// the snapshot stay untouched until wait() joins the thread, which the run
+// loop does before the next download, so rows reach the files in step order
+// and every file holds what the synchronous writer would have written. A
+// write error surfaces at the next wait().
+struct OutputThread {
+    std::thread th;
+    std::exception_ptr err;
+    std::vector<std::pair<std::string, double>> times;   // writer phases of the last snapshot
+    void start(std::function<void(OutputThread&)> fn) {
+        th = std::thread([this, fn] {
+            try {
+                fn(*this);
+            } catch (...) {
+                err = std::current_exception();
+            }
+        });
+    }
+    void timed(const char* name, const std::function<void()>& fn) {
+        const auto t0 = std::chrono::steady_clock::now();
+        fn();
+        times.emplace_back(name, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    }
+    void wait(PhaseTimes& pt) {
+        if (th.joinable()) th.join();
+        for (const auto& t : times) pt.add(t.first, t.second);
+        times.clear();
+        if (err) {
+            std::exception_ptr e = err;
+            err = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+    void wait_quietly() {
+        if (th.joinable()) th.join();
+        err = nullptr;
+        times.clear();
+    }
+    ~OutputThread() {
+        if (th.joinable()) th.join();
     }
 };
 
@@ -318,12 +368,25 @@ int run(int argc, char** argv) {
         pt.mark("metadata");
         download(m, dev.ctx);
         pt.mark("download");
-        m.log_row(log, 0);
-        pt.mark("log");
-        m.grain_log(grain_diag, grain_history, 0);
-        pt.mark("grain_log");
-        m.write_visualization(hc.out + "/fields_0");
-        pt.mark("write");
+        OutputThread writer;
+        auto output = [&](int done) {
+            writer.start([&m, &log, &grain_diag, &grain_history, &hc, done](OutputThread& w) {
+                w.timed("log", [&] {
+                    m.log_row(log, done);
+                    log.flush();
+                    if (!log) throw std::runtime_error("Diagnostics write failed.");
+                });
+                if (m.has_eta())
+                    w.timed("grain_log", [&] {
+                        m.grain_log(grain_diag, grain_history, done);
+                        grain_diag.flush();
+                        grain_history.flush();
+                        if (!grain_diag || !grain_history) throw std::runtime_error("Grain diagnostics write failed.");
+                    });
+                w.timed("write", [&] { m.write_visualization(hc.out + "/fields_" + std::to_string(done)); });
+            });
+        };
+        output(0);
 
         std::cout << "EXECUTION BACKEND: " << backend << " (HIP device step; no MPI).\n";
         std::cout << "MPI processes: 1 (X slabs)\n";
@@ -414,24 +477,18 @@ int run(int argc, char** argv) {
                               << " s" << std::endl;
                 if (done % c.output_every == 0 || done == c.steps) {
                     pt.mark("host");
+                    writer.wait(pt);
+                    pt.mark("output_wait");
                     download(m, dev.ctx);
                     pt.mark("download");
-                    m.log_row(log, done);
-                    pt.mark("log");
-                    m.grain_log(grain_diag, grain_history, done);
-                    pt.mark("grain_log");
-                    log.flush();
-                    if (!log) throw std::runtime_error("Diagnostics write failed.");
-                    if (m.has_eta()) {
-                        grain_diag.flush();
-                        grain_history.flush();
-                        if (!grain_diag || !grain_history) throw std::runtime_error("Grain diagnostics write failed.");
-                    }
-                    m.write_visualization(hc.out + "/fields_" + std::to_string(done));
-                    pt.mark("write");
+                    output(done);
                 }
             }
+            pt.mark("host");
+            writer.wait(pt);
+            pt.mark("output_wait");
         } catch (const std::exception& e) {
+            writer.wait_quietly();
             {
                 std::ofstream err(hc.out + "/STOPPED.txt");
                 err << "Last accepted time step: " << last_completed << "\n" << e.what() << "\n";

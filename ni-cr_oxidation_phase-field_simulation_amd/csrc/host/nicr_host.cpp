@@ -255,6 +255,30 @@ double HostModel::competitor_distance(std::size_t g, int owner, double x, double
     return db;
 }
 
+void HostModel::competitor_distances(std::size_t g, int owner, double x, double y, double z, double oy, double oz,
+                                      double rbest_a, double rbest_b, double& da, double& db) const {
+    da = db = std::numeric_limits<double>::infinity();
+    for (int im = -1; im <= 1; ++im)
+        for (int jm = -1; jm <= 1; ++jm) {
+            const double gy = seeds[g].y + im * ny, gz = seeds[g].z + jm * nz;
+            const double rs = std::pow(seeds[g].x - x, 2) + std::pow(gy - y, 2) + std::pow(gz - z, 2);
+            const double sep = std::sqrt(std::pow(seeds[g].x - seeds[owner].x, 2) + std::pow(gy - oy, 2) +
+                                         std::pow(gz - oz, 2));
+            if (sep > 1.e-14) {
+                da = std::min(da, std::max(0., (rs - rbest_a) / (2. * sep)));
+                db = std::min(db, std::max(0., (rs - rbest_b) / (2. * sep)));
+            }
+        }
+}
+
+// eta fields zero-filled by host threads, whole fields per thread
+void HostModel::allocate_eta() {
+    eta.assign(seeds.size(), std::vector<double>());
+    parallel_chunks(eta.size(), 1, [&](std::size_t g0, std::size_t g1) {
+        for (std::size_t g = g0; g < g1; ++g) eta[g].assign(ncell, 0.);
+    });
+}
+
 void HostModel::voronoi() {
     const nicr_config& c = hc.c;
     seeds.clear();
@@ -327,6 +351,16 @@ void HostModel::voronoi() {
     }
     areas.assign(ng, 0);
     std::mutex areas_lock;
+    // 3D with evolving grains: the eta weights (initialize_eta) come out of the
+    // same competitor pass. Each weight is stored in its eta field and the
+    // per-cell sum in eta_norm; initialize_eta then scales them by the metal
+    // fraction once the center oxide is in place.
+    const bool fuse_eta = dim == 3 && has_eta();
+    const double ell_grid = eta_ell(c) / c.dx;
+    if (fuse_eta) {
+        allocate_eta();
+        eta_norm.assign(ncell, 0.);
+    }
     // X planes on host threads; each cell depends only on the seeds
     parallel_chunks(static_cast<std::size_t>(nx), 1, [&](std::size_t ia, std::size_t ib) {
     std::vector<long long> my_areas(ng, 0);
@@ -362,6 +396,23 @@ void HostModel::voronoi() {
                                 const double sep = std::hypot(seeds[g].x - seeds[owner].x, ym - ownerY);
                                 if (sep > 1.e-14) db = std::min(db, std::max(0., (rs - rbest) / (2. * sep)));
                             }
+                } else if (fuse_eta) {
+                    // initialize_eta's rbest uses the owner image's offsets as oy - y, oz - z
+                    const double rbest_eta =
+                        std::pow(seeds[owner].x - x, 2) + std::pow(ownerY - y, 2) + std::pow(ownerZ - z, 2);
+                    double norm = 0.;
+                    for (int g = 0; g < ng; ++g) {
+                        double w = 1.;
+                        if (g != owner) {
+                            double dm, de;
+                            competitor_distances(g, owner, x, y, z, ownerY, ownerZ, rbest, rbest_eta, dm, de);
+                            db = std::min(db, dm);
+                            w = std::exp(-de / ell_grid);
+                        }
+                        norm += w;
+                        eta[g][q] = w;
+                    }
+                    eta_norm[q] = norm;
                 } else {
                     for (int g = 0; g < ng; ++g)
                         if (g != owner) db = std::min(db, competitor_distance(g, owner, x, y, z, ownerY, ownerZ, rbest));
@@ -378,7 +429,18 @@ void HostModel::initialize_eta() {
     if (seeds.size() > static_cast<std::size_t>(c.max_eta_fields))
         throw std::runtime_error(dim == 2 ? "Too many eta fields. Increase --max-eta-fields only after checking memory, or use --grain-evolution off."
                                           : "Too many eta fields; check --max-eta-fields and memory.");
-    eta.assign(seeds.size(), std::vector<double>(ncell, 0.));
+    if (dim == 3) {
+        // weights and their sums from the competitor pass in voronoi()
+        parallel_chunks(ncell, 1 << 14, [&](std::size_t q0, std::size_t q1) {
+            for (std::size_t q = q0; q < q1; ++q) {
+                const double metal = 1. - h(p[q]), norm = eta_norm[q];
+                for (std::size_t g = 0; g < eta.size(); ++g) eta[g][q] = metal * eta[g][q] / norm;
+            }
+        });
+        std::vector<double>().swap(eta_norm);
+        return;
+    }
+    allocate_eta();
     const double ell_grid = eta_ell(c) / c.dx;
     // X planes on host threads; each cell depends only on the seeds and its owner
     parallel_chunks(static_cast<std::size_t>(nx), 1, [&](std::size_t ia, std::size_t ib) {

@@ -104,6 +104,12 @@ private:
     void initialize_center_oxide();  // 2D:517-542, SL:725-751
     double competitor_distance(std::size_t g, int owner, double x, double y, double z, double oy, double oz,
                                double rbest) const;   // SL:395-406
+    // the same distance for two values of rbest, sharing each image's squared
+    // distance and seed separation
+    void competitor_distances(std::size_t g, int owner, double x, double y, double z, double oy, double oz,
+                              double rbest_a, double rbest_b, double& da, double& db) const;
+    void allocate_eta();
+    std::vector<double> eta_norm;   // 3D: per-cell sum of the eta weights, between voronoi and initialize_eta
 };
 
 long double sum_field(const std::vector<double>& a);   // Model::sum (2D:229-232)

@@ -123,3 +123,12 @@ def test_nicr3d_matches_cpu_chain(nicr, oracle, tmp_path):
         sync()
         hs.write("vtk", str(cpu / f"fields_{n}.vtk"))
         assert sha(cpu / f"fields_{n}.vtk") == sha(out / f"fields_{n}.vtk"), n
+        hs.write("log", str(cpu / "diagnostics.csv"), n)
+        hs.write("grain_log", f"{cpu / 'grain_diagnostics.csv'}|{cpu / 'grain_history.csv'}", n)
+    # the rows the program writes on its output thread, in step order: grain
+    # rows byte for byte, diagnostics within the budget tolerance above
+    for f in ("grain_diagnostics.csv", "grain_history.csv"):
+        assert open(out / f).read().splitlines()[1:] == open(cpu / f).read().splitlines(), f
+    head = open(out / "diagnostics.csv").readline()
+    (cpu / "diag_with_header.csv").write_text(head + open(cpu / "diagnostics.csv").read())
+    _close_csv(out / "diagnostics.csv", cpu / "diag_with_header.csv")
