@@ -344,8 +344,11 @@ __device__ __forceinline__ void finish_cell(const Par& P, long long c, int x, do
         double dCr = -Phys::ckCr * ((Phys::oxideCr - crv) * h64(pnv) - (Phys::oxideCr - cc) * h64(pp)) / P.dt;
         onv += dO;
         crv += dCr;
-        a.ckO = dd_add(a.ckO, dO);
-        a.ckCr = dd_add(a.ckCr, dCr);
+        // adding a zero leaves a double-double sum bitwise unchanged (its parts
+        // are never -0), so the common zero terms (no change of h(phi), no
+        // clip) skip the addition
+        if (dO != 0.) a.ckO = dd_add(a.ckO, dO);
+        if (dCr != 0.) a.ckCr = dd_add(a.ckCr, dCr);
     }
     if (x == 0) {
         if (P.enforce_layer) a.bc = dd_add(a.bc, layer_o - o_stored);
@@ -356,9 +359,11 @@ __device__ __forceinline__ void finish_cell(const Par& P, long long c, int x, do
     } else if (P.bounds_legacy) {
         const double oc = onv < 0. ? 0. : (1. < onv ? 1. : onv);
         const double ccr = crv < 0. ? 0. : (1. < crv ? 1. : crv);
-        a.clO = dd_add(a.clO, oc - onv);
-        a.clCr = dd_add(a.clCr, ccr - crv);
-        if (oc != onv || ccr != crv) a.n_clip += 1.;
+        if (oc != onv || ccr != crv) {
+            a.clO = dd_add(a.clO, oc - onv);
+            a.clCr = dd_add(a.clCr, ccr - crv);
+            a.n_clip += 1.;
+        }
         onv = oc;
         crv = ccr;
         if (pnv > .99999) pnv = 1.;
