@@ -227,6 +227,52 @@ __device__ __forceinline__ MuMob<T> mu_mob(const Par& P, const double* o7, const
     return r;
 }
 
+// The flux through the face between cells a (lower index) and b along an axis
+// of spacing hh, as div_flux writes it (2D:444-448, SL:630-636): the same
+// expression for both cells that share the face.
+__device__ __forceinline__ double face64(double Ma, double Mb, double ua, double ub, double hh) {
+    return -.5 * (Ma + Mb) * (ub - ua) / hh;
+}
+// The face fluxes of one cell and species: x-, x+, y-, y+, z-, z+ (left,
+// right, bottom, top, back, front).
+struct Faces {
+    double xm, xp, ym, yp, zm, zp;
+};
+
+// Binary64 trial of one cell from its face fluxes: the mu-BC inlet face
+// (2D:435-442,446), div_flux (2D:449, SL:637), the O and Cr updates and the phi
+// force and update (2D:556-566, SL:774-784). uo: mu_O at the cell, g_c: gb at
+// the cell (inlet face only).
+template <int DIM>
+__device__ __forceinline__ void trial_faces64(const Par& P, Faces fo, const Faces& fc, const double* p7, double oo,
+                                              double cc, int x, double S, double g_c, double uo, double& onv,
+                                              double& crv, double& pnv, double& left_flux) {
+    const double pp = p7[SC];
+    left_flux = 0.;
+    double divO = 0.;
+    if (P.oxy) {
+        if (x == 0 && P.bc == 1) {
+            const double mf = mobO64(P, P.c_ref, pp, g_c);
+            left_flux = 2. * mf * (P.mu_res - uo) / P.dx;
+            fo.xm = left_flux;
+        }
+        double d = (fo.xm - fo.xp) / P.dx + (fo.ym - fo.yp) / P.dy;
+        if (DIM == 3) d = d + (fo.zm - fo.zp) / P.dz;
+        divO = d;
+    }
+    double divC = (fc.xm - fc.xp) / P.dx + (fc.ym - fc.yp) / P.dy;
+    if (DIM == 3) divC = divC + (fc.zm - fc.zp) / P.dz;
+    onv = P.oxy ? oo + P.dt * divO : oo;
+    crv = cc + P.dt * divC;
+    const double hp = 30. * pp * pp * (pp - 1.) * (pp - 1.);
+    double lp = (p7[SXP] - 2. * pp + p7[SXM]) / (P.dx * P.dx) + (p7[SYP] - 2. * pp + p7[SYM]) / (P.dy * P.dy);
+    if (DIM == 3) lp = lp + (p7[SZP] - 2. * pp + p7[SZM]) / (P.dz * P.dz);
+    double fp = Phys::W * (4. * pp * pp * pp - 6. * pp * pp + 2. * pp)
+              + hp * (foxide64(oo, cc) - fmetal64(oo, cc)) - 2. * Phys::kp * lp;
+    if (P.has_eta) fp += 0.5 * hp * (P.etaW * (S - (1. - h64(pp))) + P.etaA * S);
+    pnv = pp - P.dt * Phys::L * fp;
+}
+
 // Trial values of one cell from mu, M at its 7 points, phi at its 7 points and
 // o, cr at the cell: div_flux with the mu-BC inlet face (2D:435-450,
 // SL:621-638), the phi force and update (2D:562-566, SL:780-784).
@@ -241,48 +287,26 @@ __device__ __forceinline__ void trial_mm(const Par& P, const MuMob<T>* m, const 
     if constexpr (PREC == PREC_FP64) {
         const double moc = m[SC].Mo, mcc = m[SC].Mc;
         const double uo = m[SC].mo, uc = m[SC].mc;
-        double divO = 0.;
+        Faces fo{0., 0., 0., 0., 0., 0.}, fc{0., 0., 0., 0., 0., 0.};
         if (P.oxy) {
-            const double right = -.5 * (moc + m[SXP].Mo) * (m[SXP].mo - uo) / P.dx;
-            double left = -.5 * (m[SXM].Mo + moc) * (uo - m[SXM].mo) / P.dx;
-            if (x == 0 && P.bc == 1) {
-                const double mf = mobO64(P, P.c_ref, pp, g_c);
-                left_flux = 2. * mf * (P.mu_res - uo) / P.dx;
-                left = left_flux;
-            }
-            const double top = -.5 * (moc + m[SYP].Mo) * (m[SYP].mo - uo) / P.dy;
-            const double bottom = -.5 * (m[SYM].Mo + moc) * (uo - m[SYM].mo) / P.dy;
-            double d = (left - right) / P.dx + (bottom - top) / P.dy;
+            fo.xm = face64(m[SXM].Mo, moc, m[SXM].mo, uo, P.dx);
+            fo.xp = face64(moc, m[SXP].Mo, uo, m[SXP].mo, P.dx);
+            fo.ym = face64(m[SYM].Mo, moc, m[SYM].mo, uo, P.dy);
+            fo.yp = face64(moc, m[SYP].Mo, uo, m[SYP].mo, P.dy);
             if (DIM == 3) {
-                const double front = -.5 * (moc + m[SZP].Mo) * (m[SZP].mo - uo) / P.dz;
-                const double back = -.5 * (m[SZM].Mo + moc) * (uo - m[SZM].mo) / P.dz;
-                d = d + (back - front) / P.dz;
+                fo.zm = face64(m[SZM].Mo, moc, m[SZM].mo, uo, P.dz);
+                fo.zp = face64(moc, m[SZP].Mo, uo, m[SZP].mo, P.dz);
             }
-            divO = d;
         }
-        double divC;
-        {
-            const double right = -.5 * (mcc + m[SXP].Mc) * (m[SXP].mc - uc) / P.dx;
-            const double left = -.5 * (m[SXM].Mc + mcc) * (uc - m[SXM].mc) / P.dx;
-            const double top = -.5 * (mcc + m[SYP].Mc) * (m[SYP].mc - uc) / P.dy;
-            const double bottom = -.5 * (m[SYM].Mc + mcc) * (uc - m[SYM].mc) / P.dy;
-            double d = (left - right) / P.dx + (bottom - top) / P.dy;
-            if (DIM == 3) {
-                const double front = -.5 * (mcc + m[SZP].Mc) * (m[SZP].mc - uc) / P.dz;
-                const double back = -.5 * (m[SZM].Mc + mcc) * (uc - m[SZM].mc) / P.dz;
-                d = d + (back - front) / P.dz;
-            }
-            divC = d;
+        fc.xm = face64(m[SXM].Mc, mcc, m[SXM].mc, uc, P.dx);
+        fc.xp = face64(mcc, m[SXP].Mc, uc, m[SXP].mc, P.dx);
+        fc.ym = face64(m[SYM].Mc, mcc, m[SYM].mc, uc, P.dy);
+        fc.yp = face64(mcc, m[SYP].Mc, uc, m[SYP].mc, P.dy);
+        if (DIM == 3) {
+            fc.zm = face64(m[SZM].Mc, mcc, m[SZM].mc, uc, P.dz);
+            fc.zp = face64(mcc, m[SZP].Mc, uc, m[SZP].mc, P.dz);
         }
-        onv = P.oxy ? oo + P.dt * divO : oo;
-        crv = cc + P.dt * divC;
-        const double hp = 30. * pp * pp * (pp - 1.) * (pp - 1.);
-        double lp = (p7[SXP] - 2. * pp + p7[SXM]) / (P.dx * P.dx) + (p7[SYP] - 2. * pp + p7[SYM]) / (P.dy * P.dy);
-        if (DIM == 3) lp = lp + (p7[SZP] - 2. * pp + p7[SZM]) / (P.dz * P.dz);
-        double fp = Phys::W * (4. * pp * pp * pp - 6. * pp * pp + 2. * pp)
-                  + hp * (foxide64(oo, cc) - fmetal64(oo, cc)) - 2. * Phys::kp * lp;
-        if (P.has_eta) fp += 0.5 * hp * (P.etaW * (S - (1. - h64(pp))) + P.etaA * S);
-        pnv = pp - P.dt * Phys::L * fp;
+        trial_faces64<DIM>(P, fo, fc, p7, oo, cc, x, S, g_c, uo, onv, crv, pnv, left_flux);
     } else {
 #pragma clang fp contract(fast)
         const float hc = h32((float)pp);
