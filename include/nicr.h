@@ -181,6 +181,27 @@ int nicr_sync(void* ctx);
  * counterpart in the reference. */
 int nicr_copy_probe(void* ctx, size_t bytes, int reps, double* ms);
 
+/* ---- initial 3D grain geometry on the device ----
+ * The per-cell distances of Model::voronoi and initialize_eta in 3D
+ * (mpi3d-slab/NiCr3D_grains.cpp:395-406,439-467,469-494), which replace the
+ * host's O(cells x seeds x 9) loops in the drop-in program's initial state.
+ * For every interior cell (1-based i, j, k; centre (i-.5, j-.5, k-.5); order
+ * ((i-1)*ny + (j-1))*nz + (k-1)):
+ *   owner  the nearest seed under the periodic Y/Z minimum image (0-based,
+ *          the first on ties);
+ *   db     the GB-mask distance: over the other seeds and their 9 Y/Z images,
+ *          min of max(0, (r_image - r_best) / (2 sep));
+ *   de[g]  with de != NULL: the same minimum for seed g with r_best from the
+ *          owner image's offsets (the eta weights' distance); 0 for the owner.
+ * Binary64 in the host code's operation order, so every value is bitwise the
+ * host's; exp() of the mask and of the weights stays with the caller (glibc
+ * libm defines them). seeds: ng (x, y, z) in grid units. owner, db: ncell
+ * entries; de: ng pointers to ncell doubles, or NULL. Uses the current device
+ * and at most about 2 GiB of device memory. No counterpart call in the
+ * reference. */
+int nicr_grain_distances(int nx, int ny, int nz, int ng, const double* seeds, int32_t* owner, double* db,
+                         double* const* de);
+
 /* ---- ensembles of 2D instances ----
  * No counterpart in the reference, whose parameter sweeps are separate runs of
  * the 2D program (one Model per process, mpi2d:1148-1236). An ensemble holds n

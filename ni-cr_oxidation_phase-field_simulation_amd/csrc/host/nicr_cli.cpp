@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <exception>
 #include <filesystem>
 #include <functional>
@@ -334,6 +335,10 @@ int run(int argc, char** argv) {
         }
         if (fs::exists(hc.out) && !fs::is_empty(hc.out)) throw std::runtime_error("Output directory is not empty; choose a new --out.");
         fs::create_directories(hc.out);
+        // the 3D grain distances of the initial state on the GPU (bitwise the
+        // host loops, which NICR_HOST_INIT=host selects)
+        const char* init = std::getenv("NICR_HOST_INIT");
+        hc.device_init = !(init && std::strcmp(init, "host") == 0);
         HostModel m(hc);
         pt.mark("init");
         const nicr_config& c = hc.c;

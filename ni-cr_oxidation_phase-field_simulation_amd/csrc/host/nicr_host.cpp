@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <array>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
@@ -360,6 +361,46 @@ void HostModel::voronoi() {
     if (fuse_eta) {
         allocate_eta();
         eta_norm.assign(ncell, 0.);
+    }
+    if (dim == 3 && hc.device_init) {
+        // owners and distances from the device (bitwise the loop below); the
+        // exp() of the mask and of the weights here, as below
+        std::vector<double> sxyz(3 * std::size_t(ng));
+        for (int g = 0; g < ng; ++g) {
+            sxyz[3 * g] = seeds[g].x;
+            sxyz[3 * g + 1] = seeds[g].y;
+            sxyz[3 * g + 2] = seeds[g].z;
+        }
+        std::vector<int32_t> own(ncell);
+        std::vector<double> dist(ncell);
+        std::vector<double*> de;
+        if (fuse_eta)
+            for (auto& e : eta) de.push_back(e.data());
+        if (nicr_grain_distances(nx, ny, nz, ng, sxyz.data(), own.data(), dist.data(), fuse_eta ? de.data() : nullptr) !=
+            NICR_OK)
+            throw std::runtime_error(std::string("Device grain geometry failed: ") + nicr_last_error());
+        parallel_chunks(ncell, 1 << 14, [&](std::size_t q0, std::size_t q1) {
+            std::vector<long long> my_areas(ng, 0);
+            for (std::size_t q = q0; q < q1; ++q) {
+                const int owner = own[q];
+                grain[q] = owner + 1;
+                ++my_areas[owner];
+                const double db = dist[q];
+                gb[q] = ng == 1 ? 0. : std::exp(-4. * std::log(2.) * db * db / (c.gb_fwhm_grid * c.gb_fwhm_grid));
+                if (fuse_eta) {
+                    double norm = 0.;
+                    for (int g = 0; g < ng; ++g) {
+                        const double w = g == owner ? 1. : std::exp(-eta[g][q] / ell_grid);
+                        norm += w;
+                        eta[g][q] = w;
+                    }
+                    eta_norm[q] = norm;
+                }
+            }
+            std::lock_guard<std::mutex> guard(areas_lock);
+            for (int g = 0; g < ng; ++g) areas[g] += my_areas[g];
+        });
+        return;
     }
     // X planes on host threads; each cell depends only on the seeds
     parallel_chunks(static_cast<std::size_t>(nx), 1, [&](std::size_t ia, std::size_t ib) {
@@ -1093,6 +1134,10 @@ int nicr_host_create(const nicr_config* cfg, const char* bc, const char* nucleat
         hc.c = *cfg;
         hc.bc = bc;
         hc.nucleation = nucleation;
+        // the host loops by default; NICR_HOST_INIT=device takes the 3D grain
+        // distances from the GPU
+        const char* init = std::getenv("NICR_HOST_INIT");
+        hc.device_init = init && std::strcmp(init, "device") == 0;
         nicr_config_derive(&hc.c);
         auto* s = new nicr_host_state{new nicrh::HostModel(hc)};
         *out = s;

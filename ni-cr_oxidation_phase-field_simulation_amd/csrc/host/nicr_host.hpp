@@ -41,6 +41,7 @@ struct HostConfig {
     std::string precision = "fp64";                                    // this build: fp64 | mixed
     bool self_test = false;
     int dim = 3;
+    bool device_init = false;   // 3D grain distances of the initial state on the GPU (nicr_grain_distances)
     bool has_eta() const { return c.grain_evolution && c.grains; }
 };
 

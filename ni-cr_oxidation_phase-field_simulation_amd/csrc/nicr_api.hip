@@ -1105,5 +1105,6 @@ int nicr_sync(void* h) {
 
 }  // extern "C"
 
+#include "nicr_init.hip"
 #include "nicr_batch.hip"
 #include "nicr_nn.hip"
