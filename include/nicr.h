@@ -181,6 +181,17 @@ int nicr_sync(void* ctx);
  * counterpart in the reference. */
 int nicr_copy_probe(void* ctx, size_t bytes, int reps, double* ms);
 
+/* Per-cell parts of the 3D grain rows (Model::grain_log and grain_energy,
+ * mpi3d-slab/NiCr3D_grains.cpp:546-590) from the accepted device state.
+ * energy_terms (ncell): each cell's discrete grain-energy density times the
+ * cell volume, binary64 in the host code's operation order, so a long double
+ * sum in cell order is the reference's grain energy. extrema: min and max of
+ * every eta value, min and max of sum eta^2, max of the GB mask. counts
+ * (n_eta + 1): [0] metal cells (phi < 0.5) with no dominant grain, [g] cells
+ * whose dominant grain is g (largest eta above 1e-12, first on ties, phi <
+ * 0.5). 3D contexts with eta fields only (at most 256). */
+int nicr_grain_terms(void* ctx, double* energy_terms, double extrema[5], int64_t* counts);
+
 /* ---- initial 3D grain geometry on the device ----
  * The per-cell distances of Model::voronoi and initialize_eta in 3D
  * (mpi3d-slab/NiCr3D_grains.cpp:395-406,439-467,469-494), which replace the

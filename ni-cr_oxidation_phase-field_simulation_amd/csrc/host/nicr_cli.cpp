@@ -159,6 +159,16 @@ void download(HostModel& m, void* ctx) {
         check(nicr_download_field(ctx, NICR_FIELD_ETA_S2, m.eta_s2.data()), "download eta_s2");
         check(nicr_download_field(ctx, NICR_FIELD_ETA_PAIRS, m.eta_pairs.data()), "download eta_pairs");
     }
+    // the grain rows' per-cell terms of this snapshot, from the device
+    // (NICR_GRAIN_TERMS=host keeps the host loops)
+    m.grain_terms_valid = false;
+    const char* gt = std::getenv("NICR_GRAIN_TERMS");
+    if (m.has_eta() && m.dim == 3 && m.eta.size() <= 256 && !(gt && std::strcmp(gt, "host") == 0)) {
+        m.grain_terms.resize(m.ncell);
+        m.grain_counts.resize(m.eta.size() + 1);
+        check(nicr_grain_terms(ctx, m.grain_terms.data(), m.grain_extrema, m.grain_counts.data()), "grain terms");
+        m.grain_terms_valid = true;
+    }
     nicr_budget b{};
     check(nicr_get_budget(ctx, &b), "budget");
     m.b.nucO = join(b.nucO);

@@ -581,6 +581,11 @@ static double local_energy(double S, double P, double phi, const nicr_config& c)
 
 long double HostModel::grain_energy() const {
     if (!has_eta()) return 0.;
+    if (grain_terms_valid && dim == 3) {
+        long double local = 0.;
+        for (std::size_t q = 0; q < ncell; ++q) local += grain_terms[q];
+        return local;
+    }
     const nicr_config& c = hc.c;
     // each cell's term (a double) on the host threads, then the long double
     // sum in the serial loop order, so the total is bitwise that of one thread
@@ -624,8 +629,22 @@ void HostModel::grain_log(std::ostream& f, std::ostream& history, int n) const {
     double lo = 1., hi = 0., slo = std::numeric_limits<double>::infinity(), shi = 0., gbmax = 0.;
     int unassigned = 0;
     std::vector<double> metal(ncell);
-    // extrema and counts do not depend on the order: per-thread partials, merged
+    // extrema and counts do not depend on the order: from the device, or
+    // per-thread partials, merged
     std::mutex merge;
+    const bool dev = grain_terms_valid && dim == 3;
+    if (dev) {
+        lo = grain_extrema[0];
+        hi = grain_extrema[1];
+        slo = grain_extrema[2];
+        shi = grain_extrema[3];
+        gbmax = grain_extrema[4];
+        unassigned = int(grain_counts[0]);
+        for (std::size_t g = 0; g < ng; ++g) cells[g] = int(grain_counts[g + 1]);
+        parallel_chunks(ncell, 1 << 14, [&](std::size_t q0, std::size_t q1) {
+            for (std::size_t q = q0; q < q1; ++q) metal[q] = 1. - h(p[q]);
+        });
+    } else
     parallel_chunks(ncell, 1 << 14, [&](std::size_t q0, std::size_t q1) {
         double tlo = 1., thi = 0., tslo = std::numeric_limits<double>::infinity(), tshi = 0., tgb = 0.;
         int tun = 0;

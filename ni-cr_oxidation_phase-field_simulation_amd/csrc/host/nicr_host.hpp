@@ -79,6 +79,13 @@ public:
     LBudget b;
     long double initialO = 0, initialCr = 0;
     int center_seed_cells = 0;
+    // The snapshot's per-cell grain-energy terms and order-free grain
+    // statistics computed on the device (nicr_grain_terms); grain_log uses
+    // them while grain_terms_valid, and its own loops otherwise.
+    std::vector<double> grain_terms;
+    double grain_extrema[5] = {0., 0., 0., 0., 0.};
+    std::vector<int64_t> grain_counts;
+    bool grain_terms_valid = false;
 
     explicit HostModel(const HostConfig& cfg);  // Model::Model (2D:209-228, SL:356-381)
     bool has_eta() const { return hc.has_eta(); }

@@ -67,6 +67,9 @@ struct Ctx {
     double* out4[4] = {nullptr, nullptr, nullptr, nullptr};  // refresh outputs muO, muCr, mo, mc
     double* dev_obuf = nullptr;  // device copy of {o0, o1, cr0, cr1, p0, p1}
     float* stage = nullptr;      // pinned host copy of one binary32 field (first conversion on)
+    double* grain_term = nullptr;  // nicr_grain_terms scratch (allocated on first use)
+    void* grain_part = nullptr;
+    void* grain_counts = nullptr;
     int* nuc_tiles = nullptr;    // nucleation scratch (allocated on first use)
     int* nuc_planes = nullptr;
     int* nuc_list = nullptr;
@@ -1106,5 +1109,6 @@ int nicr_sync(void* h) {
 }  // extern "C"
 
 #include "nicr_init.hip"
+#include "nicr_grains.hip"
 #include "nicr_batch.hip"
 #include "nicr_nn.hip"
