@@ -60,3 +60,25 @@ def test_surrogate_struct_layout_matches_c(nicr, tmp_path):
             sur.SurOptions.settle_steps.offset, ctypes.sizeof(sur.SurReport), sur.SurReport.last_residual.offset,
             sur.SurReport.step.offset]
     assert got == want
+
+
+def test_grain_entry_points_reject_bad_arguments_before_the_device():
+    """nicr_grain_distances and nicr_grain_terms check their arguments before any
+    HIP call (the error path runs without a GPU)."""
+    so = ctypes.CDLL(os.path.join(PKG, "libnicr.so"))
+    so.nicr_last_error.restype = ctypes.c_char_p
+    fn = so.nicr_grain_distances
+    fn.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 4
+    seeds = (ctypes.c_double * 3)(1.0, 1.0, 1.0)
+    owner = (ctypes.c_int32 * 8)()
+    db = (ctypes.c_double * 8)()
+    assert fn(0, 2, 2, 1, seeds, owner, db, None) == -1
+    assert b"bad argument" in so.nicr_last_error()
+    assert fn(2, 2, 2, 0, seeds, owner, db, None) == -1
+    assert fn(2, 2, 2, 1, None, owner, db, None) == -1
+    gt = so.nicr_grain_terms
+    gt.argtypes = [ctypes.c_void_p] * 4
+    ext = (ctypes.c_double * 5)()
+    cnt = (ctypes.c_int64 * 4)()
+    assert gt(None, db, ext, cnt) == -1
+    assert b"null argument" in so.nicr_last_error()
