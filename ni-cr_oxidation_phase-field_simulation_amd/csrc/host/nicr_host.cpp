@@ -1017,19 +1017,28 @@ class VtiWriter {
     void blocks(Fn value) {
         const std::uint64_t n = std::uint64_t(m.nx) * m.ny * m.nz, total = n * sizeof(T);
         const std::uint64_t nb = (total + kBlock - 1) / kBlock;
-        // the array in X-fastest order, gathered by worker threads over (z, y) rows
+        // the array in X-fastest order (the storage is Z-fastest), transposed by
+        // worker threads in 64 x 64 (x, z) tiles of one y, so reads and writes
+        // both run along 64 contiguous elements
         std::vector<T> all(static_cast<std::size_t>(n));
-        const std::size_t nrow = static_cast<std::size_t>(m.nz) * m.ny;
-        parallel_chunks(nrow, 64, [&](std::size_t a, std::size_t b) {
-            for (std::size_t r = a; r < b; ++r) {
-                const int k = static_cast<int>(r / m.ny) + 1, j = static_cast<int>(r % m.ny) + 1;
-                T* row = all.data() + r * m.nx;
-                for (int i = 1; i <= m.nx; ++i) {
-                    const double v = value(m.at(i, j, k));
-                    if (std::isfinite(v) && std::abs(v) > std::numeric_limits<T>::max())
-                        throw std::runtime_error(std::string("VTI ") + kNames[next] +
-                                                 " exceeds output type range; use --output-precision float64.");
-                    row[i - 1] = static_cast<T>(v);
+        constexpr int TT = 64;
+        const int nkt = (m.nz + TT - 1) / TT;
+        const std::size_t rowlen = static_cast<std::size_t>(m.nx), zstride = static_cast<std::size_t>(m.ny) * m.nx;
+        parallel_chunks(static_cast<std::size_t>(m.ny) * nkt, 1, [&](std::size_t a, std::size_t b) {
+            for (std::size_t t = a; t < b; ++t) {
+                const int j = static_cast<int>(t / nkt) + 1, k0 = static_cast<int>(t % nkt) * TT + 1;
+                const int k1 = std::min(m.nz, k0 + TT - 1);
+                for (int i0 = 1; i0 <= m.nx; i0 += TT) {
+                    const int i1 = std::min(m.nx, i0 + TT - 1);
+                    for (int i = i0; i <= i1; ++i)
+                        for (int k = k0; k <= k1; ++k) {
+                            const double v = value(m.at(i, j, k));
+                            if (std::isfinite(v) && std::abs(v) > std::numeric_limits<T>::max())
+                                throw std::runtime_error(std::string("VTI ") + kNames[next] +
+                                                         " exceeds output type range; use --output-precision float64.");
+                            all[(k - 1) * zstride + static_cast<std::size_t>(j - 1) * rowlen + (i - 1)] =
+                                static_cast<T>(v);
+                        }
                 }
             }
         });
@@ -1126,7 +1135,12 @@ void HostModel::write_vti(const std::string& path) const {
     w.array([&](std::size_t q) { return cr[q]; });
     w.array([&](std::size_t q) { return muO[q]; });
     w.array([&](std::size_t q) { return muCr[q]; });
-    w.array([&](std::size_t q) { return grain_sum(q); });
+    // grain_sum reads every eta field at the cell: evaluated in storage order
+    std::vector<double> gsum(ncell);
+    parallel_chunks(ncell, 1 << 14, [&](std::size_t q0, std::size_t q1) {
+        for (std::size_t q = q0; q < q1; ++q) gsum[q] = grain_sum(q);
+    });
+    w.array([&](std::size_t q) { return gsum[q]; });
     w.array([&](std::size_t q) { return gb[q]; });
     w.finish();
 }
