@@ -983,7 +983,12 @@ void HostModel::write_vtk(const std::string& path) const {
         write("conc_Cr", [&](std::size_t q) { return cr[q]; });
         write("mu_O", [&](std::size_t q) { return muO[q]; });
         write("mu_Cr", [&](std::size_t q) { return muCr[q]; });
-        write("grain_sum", [&](std::size_t q) { return grain_sum(q); });
+        // grain_sum reads every eta field at the cell: evaluated in storage order
+        std::vector<double> gsum(ncell);
+        parallel_chunks(ncell, 1 << 14, [&](std::size_t q0, std::size_t q1) {
+            for (std::size_t q = q0; q < q1; ++q) gsum[q] = grain_sum(q);
+        });
+        write("grain_sum", [&](std::size_t q) { return gsum[q]; });
         write("GB_mask", [&](std::size_t q) { return gb[q]; });
     }
     if (!f) throw std::runtime_error("Output write failed: " + path);
