@@ -222,6 +222,7 @@ HostModel::HostModel(const HostConfig& cfg) : hc(cfg) {
     grain.assign(ncell, 0.);
     voronoi();
     if (dim == 2) gb_initial = gb;
+    const bool layer = hc.bc == "concentration";
     for (int i = 1; i <= nx; ++i)
         for (int j = 1; j <= ny; ++j)
             for (int k = 1; k <= nz; ++k) {
@@ -229,7 +230,7 @@ HostModel::HostModel(const HostConfig& cfg) : hc(cfg) {
                 o[q] = Phys::bulkO;
                 cr[q] = Phys::bulkCr;
                 p[q] = 0.;
-                if (hc.bc == "concentration" && i == 1) {
+                if (layer && i == 1) {
                     o[q] = Phys::layerO;
                     cr[q] = Phys::layerCr;
                 }
