@@ -137,7 +137,7 @@ void upload(const HostModel& m, void* ctx) {
 }
 
 // Model::refresh + the fields every writer reads (2D:600, SL:830).
-void download(HostModel& m, void* ctx) {
+void download(HostModel& m, void* ctx, bool grain_terms = true) {
     check(nicr_download_field(ctx, NICR_FIELD_O, m.o.data()), "download o");
     check(nicr_download_field(ctx, NICR_FIELD_CR, m.cr.data()), "download cr");
     check(nicr_download_field(ctx, NICR_FIELD_PHI, m.p.data()), "download phi");
@@ -163,7 +163,7 @@ void download(HostModel& m, void* ctx) {
     // (NICR_GRAIN_TERMS=host keeps the host loops)
     m.grain_terms_valid = false;
     const char* gt = std::getenv("NICR_GRAIN_TERMS");
-    if (m.has_eta() && m.dim == 3 && m.eta.size() <= 256 && !(gt && std::strcmp(gt, "host") == 0)) {
+    if (grain_terms && m.has_eta() && m.dim == 3 && m.eta.size() <= 256 && !(gt && std::strcmp(gt, "host") == 0)) {
         m.grain_terms.resize(m.ncell);
         m.grain_counts.resize(m.eta.size() + 1);
         check(nicr_grain_terms(ctx, m.grain_terms.data(), m.grain_extrema, m.grain_counts.data()), "grain terms");
@@ -508,7 +508,7 @@ int run(int argc, char** argv) {
                 std::ofstream err(hc.out + "/STOPPED.txt");
                 err << "Last accepted time step: " << last_completed << "\n" << e.what() << "\n";
             }
-            download(m, dev.ctx);
+            download(m, dev.ctx, false);   // the fields only: no grain rows follow
             m.write_visualization(hc.out + "/state_at_failure");
             throw;
         }
