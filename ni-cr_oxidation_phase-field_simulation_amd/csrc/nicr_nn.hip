@@ -314,7 +314,7 @@ __global__ __launch_bounds__(256) void k_sur_trap(long long n, const double* __r
 // min(-eta_h) per block (columns 0, 1).
 __global__ __launch_bounds__(256) void k_sur_eta_pre(long long ncell, long long fs, int n_eta, double J,
                                                      double eta_scale, const double* __restrict__ prev,
-                                                     double* __restrict__ cur, double* __restrict__ bk,
+                                                     const double* __restrict__ cur, double* __restrict__ out,
                                                      double* __restrict__ s2, double* __restrict__ pairs,
                                                      double* __restrict__ gb, double* __restrict__ bk_mask,
                                                      double* __restrict__ part) {
@@ -329,8 +329,7 @@ __global__ __launch_bounds__(256) void k_sur_eta_pre(long long ncell, long long 
             const long long k = (long long)g * fs + i;
             const double a = prev[k], b = cur[k];
             const double h = b + J * (b - a);
-            bk[k] = b;
-            cur[k] = h;
+            out[k] = h;
             mn = fmin(mn, h);
             mx = fmin(mx, -h);
             const double q = h * h;
@@ -1560,6 +1559,30 @@ static int sur_partials(Net* n, int nv, long double* sum, int first_min = 8) {
     return NICR_OK;
 }
 
+// During nicr_sur_advance the net's spare eta buffer trades places with the
+// context's current one at every jump. On every exit the context gets its own
+// two buffers back (the state copied over if it sits in the spare) and the
+// net its spare, so neither holds the other's memory between calls.
+struct SpareEta {
+    Net* n;
+    Ctx* c;
+    void*& spare;
+    void* own0;
+    void* own1;
+    size_t bytes;
+    ~SpareEta() {
+        if (!bytes) return;
+        for (int i = 0; i < 2; ++i)
+            if (c->eta[i] != own0 && c->eta[i] != own1) {
+                (void)hipStreamSynchronize(n->stream);
+                (void)hipStreamSynchronize(c->stream);
+                (void)hipMemcpy(spare, c->eta[i], bytes, hipMemcpyDeviceToDevice);
+                std::swap(c->eta[i], spare);
+            }
+        n->bk_eta = static_cast<double*>(spare);
+    }
+};
+
 int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* opt, nicr_sur_report* rep) {
     Net* n = static_cast<Net*>(h);
     Ctx* c = static_cast<Ctx*>(model);
@@ -1575,6 +1598,8 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
     const bool evo = c->eta_evolves();
     if (evo && c->te != 8) return fail(NICR_E_INVALID, "nicr_sur_advance: eta must be binary64");
     if (evo && (rc = sur_eta_buffers(n, c))) return rc;
+    void* spare = n->bk_eta;
+    const SpareEta restore{n, c, spare, c->eta[0], c->eta[1], evo ? (size_t)c->cfg.n_eta * c->P.fstride * 8 : 0};
     const long long V = n->V(0);
     *rep = nicr_sur_report{};
     const bool conserve = opt->conserve_mass != 0;
@@ -1654,13 +1679,17 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         bool ok = std::isfinite(s1[0]) && std::isfinite(s1[1]) && omin >= lo && omax <= hi && cmin >= lo &&
                   cmax <= hi && pmin >= lo && pmax <= hi;
         if (evo) {
-            // eta_h and its mask in the current buffers; eta_-1 (the other
-            // buffer) is overwritten by the check step
+            // eta_h into the spare buffer, which then trades places with the
+            // current one: the context steps from eta_h and the spare holds
+            // eta_0 for the residual and a rollback, with no copy. Its mask in
+            // the current mask buffers (the old one backed up); eta_-1 (the
+            // other buffer) is overwritten by the check step
             const Par& P = c->P;
             hipLaunchKernelGGL(k_sur_eta_pre, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, c->ncell, P.fstride,
                                c->cfg.n_eta, (double)J, P.eta_scale, (const double*)c->eta[cur ^ 1],
-                               (double*)c->eta[cur], n->bk_eta, (double*)c->s2[cur], (double*)c->pairs[cur],
-                               (double*)c->gb[cur], n->bk_mask, n->part);
+                               (const double*)c->eta[cur], (double*)spare, (double*)c->s2[cur],
+                               (double*)c->pairs[cur], (double*)c->gb[cur], n->bk_mask, n->part);
+            std::swap(c->eta[cur], spare);
             long double pe[2];
             if ((rc = sur_partials(n, 2, pe, 0))) return rc;
             ok = ok && (double)pe[0] >= lo && -(double)pe[1] <= hi;
@@ -1682,7 +1711,7 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
             if (evo) {
                 hipLaunchKernelGGL(k_sur_eta_trap, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, c->ncell,
                                    c->P.fstride, c->cfg.n_eta, (double)J, (const double*)c->eta[k ^ 1],
-                                   (const double*)c->eta[k], n->bk_eta, n->part);
+                                   (const double*)c->eta[k], (const double*)spare, n->part);
                 long double pr[2];
                 if ((rc = sur_partials(n, 2, pr))) return rc;
                 reta = pr[1] > 0 ? (double)std::sqrt(pr[0] / pr[1]) : (pr[0] == 0 ? 0. : INFINITY);
@@ -1719,8 +1748,7 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         HIPCHK(hipMemcpyAsync(c->cr[cb], n->bk[1], V * 8, hipMemcpyDeviceToDevice, n->stream));
         HIPCHK(hipMemcpyAsync(c->p[cb], n->bk[2], V * 8, hipMemcpyDeviceToDevice, n->stream));
         if (evo) {
-            HIPCHK(hipMemcpyAsync(c->eta[cb], n->bk_eta, (size_t)c->cfg.n_eta * c->P.fstride * 8,
-                                  hipMemcpyDeviceToDevice, n->stream));
+            std::swap(c->eta[cb], spare);   // eta_0 back into the context
             HIPCHK(hipMemcpyAsync(c->s2[cb], n->bk_mask, V * 8, hipMemcpyDeviceToDevice, n->stream));
             HIPCHK(hipMemcpyAsync(c->pairs[cb], n->bk_mask + V, V * 8, hipMemcpyDeviceToDevice, n->stream));
             HIPCHK(hipMemcpyAsync(c->gb[cb], n->bk_mask + 2 * V, V * 8, hipMemcpyDeviceToDevice, n->stream));
