@@ -396,7 +396,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the X-slab sharding design is described in DESIGN.md).", file=sys.stderr)
+              "(several GPUs would each hold an X slab of the mesh, which is not built).", file=sys.stderr)
         sys.exit(2)
     nicr = _pkg()
     if args.copy_probe_only:
