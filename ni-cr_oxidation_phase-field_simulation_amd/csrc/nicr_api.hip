@@ -219,8 +219,8 @@ bool eta_rows_apply(const Ctx* c, int ng, size_t te) {
     return (2 * ng) % ty == 0 && (size_t)ng * (ty + 2) * nz * te <= 160 * 1024;
 }
 
-// X-marching eta kernel for 3D meshes with up to 32 eta fields: one pass over
-// fields g0 .. g0+NG-1.
+// X-marching eta kernel for 3D meshes: one pass over fields g0 .. g0+NG-1
+// (NG <= 32; more fields take several passes).
 template <int PREC, int NG, int TY, class TE, class TM>
 void launch_eta_march_ng(Ctx* c, const Bufs<TE, TM>& B, int g0) {
     const Par& P = c->P;
@@ -287,7 +287,7 @@ bool launch_eta_march(Ctx* c, const Bufs<TE, TM>& B) {
     // the kernel addresses the fields of one pass with 32-bit byte offsets
     int ng_max = 1;
     while (ng_max * 2 <= std::min(n, cap) && ng_max < 32) ng_max *= 2;
-    if (c->march_disabled || c->dim != 3 || n > 32 || c->P.nz < 32 ||
+    if (c->march_disabled || c->dim != 3 || c->P.nz < 32 ||
         (unsigned long long)c->P.fstride * sizeof(TE) * (unsigned long long)ng_max >= (1ull << 32))
         return false;
     if constexpr (PREC == PREC_FP64) {

@@ -92,6 +92,11 @@ CASES = [
     (3, (6, 10, 128), 12, True, dict(bc="concentration", bounds="legacy")),
     (3, (4, 3, 256), 18, True, dict(bc="mu", bounds="legacy", dt=2e-10)),
     (3, (5, 2, 64), 32, True, dict(bc="closed", bounds="legacy", ck="off")),
+    # more than 32 fields: several marching passes (the reference's one field
+    # per grain, e.g. 64 grains at 256^3)
+    (3, (5, 4, 64), 40, True, dict(bc="concentration", bounds="legacy")),
+    (3, (6, 5, 40), 37, True, dict(bc="mu", bounds="legacy", dt=2e-10)),
+    (3, (3, 2, 256), 64, True, dict(bc="closed", bounds="legacy", ck="off")),
 ]
 
 
@@ -215,7 +220,7 @@ def test_determinism(nicr):
 
 
 @pytest.mark.parametrize("shape,n_eta", [((6, 9, 70), 5), ((7, 3, 40), 9), ((12, 10, 64), 32), ((5, 5, 128), 12),
-                                         ((4, 7, 256), 18)])
+                                         ((4, 7, 256), 18), ((5, 6, 40), 37), ((4, 3, 256), 64)])
 def test_march_equals_basic_kernel(nicr, shape, n_eta, monkeypatch):
     """The X-marching eta kernel and the unblocked kernel give the same fields
     (bitwise in FP64; to binary32 rounding in the mixed path)."""
