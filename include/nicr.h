@@ -159,6 +159,12 @@ int nicr_step(void* ctx, int nsteps, nicr_step_report* report);
 int nicr_nucleate(void* ctx, int* inserted_cells);
 /* Model::stamp (mpi2d:503-516, mpi3d-slab:704-724); coordinates are 1-based. */
 int nicr_stamp(void* ctx, int x, int y, int z, int* changed);
+/* Model::initialize_center_oxide (mpi2d:517-542, mpi3d-slab:725-751): one hard
+ * oxide disk/sphere of radius cfg.radius_grid about the box centre, non-periodic,
+ * on the current state; inserted O/Cr go to nucO/nucCr and nuclei becomes 1.
+ * Errors with the reference's messages: radius >= (min side - 1)/2, a nucleus
+ * already recorded, or no cell centre covered (the state is then unchanged). */
+int nicr_center_seed(void* ctx, int* cells);
 
 /* ---- diagnostics ---- */
 /* Model::sum plus extrema of one interior field. */

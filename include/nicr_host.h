@@ -45,6 +45,11 @@ int nicr_host_set_budget(void* state, const nicr_budget* b);
 int nicr_host_write(void* state, int what, const char* path, int step, const char* options);
 /* Seed coordinates (3 doubles per seed). */
 int nicr_host_seeds(void* state, double* xyz);
+/* n draws of std::uniform_real_distribution<double>(lo, hi) from
+ * std::mt19937_64(seed), in order (the seeding of mpi3d-slab:425-427): the
+ * composition noise of the benchmark states, drawn in the reference's loop order
+ * when out[] is filled in storage order. No counterpart in the reference. */
+int nicr_host_uniform_noise(unsigned long long seed, size_t n, double lo, double hi, double* out);
 
 #ifdef __cplusplus
 }

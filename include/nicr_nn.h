@@ -138,7 +138,9 @@ typedef struct nicr_sur_report {
  * ||R|| <= residual_tol ||xh - x0|| (an O or Cr proposal that does not move
  * fails). Otherwise the model is put back exactly to x0 (fields, eta, mask,
  * state index, budgets) and takes fallback_steps (default `jump`) plain steps
- * instead. */
+ * instead. Only models whose plain step takes no O or Cr from outside can jump:
+ * a model with an open O boundary (bc concentration or mu with O diffusion on),
+ * legacy Ck or Ksp nucleation is refused with NICR_E_INVALID and left unchanged. */
 int nicr_sur_advance(void* net, void* model, int nsteps, const nicr_sur_options* opt, nicr_sur_report* report);
 
 #ifdef __cplusplus

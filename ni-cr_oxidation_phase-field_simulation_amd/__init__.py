@@ -24,7 +24,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libnicr.so")
 
 # ---- constants mirrored from include/nicr.h ----
-ABI_VERSION = 1
+ABI_VERSION = 2
 BC = {"concentration": 0, "mu": 1, "closed": 2}
 CK = {"legacy": 0, "off": 1}
 BOUNDS = {"stop": 0, "legacy": 1}
@@ -97,6 +97,7 @@ def _declare(lib, prefix: str):
         "step": ([_VOIDP, ctypes.c_int, ctypes.POINTER(NicrStepReport)], ctypes.c_int),
         "nucleate": ([_VOIDP, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "stamp": ([_VOIDP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "center_seed": ([_VOIDP, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "last_error": ([], ctypes.c_char_p),
     }
     for name, (args, res) in sigs.items():
@@ -312,6 +313,13 @@ class Model:
     def stamp(self, x: int, y: int, z: int = 0) -> int:
         n = ctypes.c_int(0)
         self._h.check(self._h._fn("stamp")(self._h.h, int(x), int(y), int(z), ctypes.byref(n)))
+        return n.value
+
+    def initialize_center_oxide(self) -> int:
+        """The centre seed (Model::initialize_center_oxide); returns the cell count.
+        Raises RuntimeError with the reference's message when it is refused."""
+        n = ctypes.c_int(0)
+        self._h.check(self._h._fn("center_seed")(self._h.h, ctypes.byref(n)))
         return n.value
 
     # ---- diagnostics (GPU reductions) ----

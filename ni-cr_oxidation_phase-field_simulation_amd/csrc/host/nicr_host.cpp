@@ -1345,4 +1345,20 @@ int nicr_host_seeds(void* h, double* xyz) {
     for (const auto& s : m.seeds) { xyz[q++] = s.x; xyz[q++] = s.y; xyz[q++] = s.z; }
     return 0;
 }
+// Seeded composition noise of the benchmark workloads: n draws of
+// std::uniform_real_distribution<double>(lo, hi) from std::mt19937_64(seed), in
+// order. The reference seeds its Voronoi points this way (mpi3d-slab:425-427);
+// filling out[] in storage order draws in the reference's loop order (i, then j,
+// then k), so the same noise can be rebuilt by any program linked against the
+// same C++ library.
+int nicr_host_uniform_noise(unsigned long long seed, size_t n, double lo, double hi, double* out) {
+    if (!out && n) {
+        g_host_error = "nicr_host_uniform_noise: null output";
+        return -1;
+    }
+    std::mt19937_64 gen(seed);
+    std::uniform_real_distribution<double> u(lo, hi);
+    for (size_t q = 0; q < n; ++q) out[q] = u(gen);
+    return 0;
+}
 }

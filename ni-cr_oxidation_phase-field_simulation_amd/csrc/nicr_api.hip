@@ -9,6 +9,7 @@
 #include "nicr_nucleate.hip"
 #include "host_threads.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -88,6 +89,7 @@ struct Ctx {
     int eta_nseg = 0;              // X segments per column (NICR_ETA_NSEG; 0: one wave of resident blocks)
     int eta_ng = 16;               // max eta fields per marching pass (NICR_ETA_NG: 8, 16, 32)
     bool eta_rows_disabled = false;  // NICR_ETA_ROWS=0 keeps the 64-wide tiles where whole rows would fit
+    std::string launch_error;        // set-up failure of a marching kernel (reported by nicr_step)
     size_t bytes = 0;
     std::vector<void*> allocs;
     bool timing = false;
@@ -184,6 +186,30 @@ static int pick_nseg(int tiles, int resident, int nx, double lead) {
     return best;
 }
 
+// Raise a marching kernel's dynamic LDS limit and count the blocks the chip
+// holds at once. Every HIP result is checked: a launch whose LDS the runtime
+// refused would otherwise be enqueued with a size it cannot run, so a failure
+// is kept in c->launch_error and reported by the calling entry point.
+static int prepare_kernel(Ctx* c, const void* fn, int threads, size_t lds) {
+    const size_t lds_cap = 160 * 1024;
+    if (lds > lds_cap || threads < 64 || threads > 1024) {
+        c->launch_error = "kernel launch shape out of range (threads " + std::to_string(threads) + ", LDS " +
+                          std::to_string(lds) + " bytes)";
+        return 0;
+    }
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int per_cu = 0, ncu = 0, dev = 0;
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds);
+    if (e != hipSuccess || per_cu < 1 || ncu < 1) {
+        c->launch_error = std::string("kernel set-up failed: ") +
+                          (e != hipSuccess ? hipGetErrorString(e) : "no block fits a compute unit");
+        return 0;
+    }
+    return per_cu * ncu;
+}
+
 // Whole-row variant (k_eta_rows): blocks of 512 / nz rows of nz cells.
 template <int PREC, int NG, int NZ, class TE, class TM>
 void launch_eta_rows_ng(Ctx* c, const Bufs<TE, TM>& B, int g0) {
@@ -192,15 +218,8 @@ void launch_eta_rows_ng(Ctx* c, const Bufs<TE, TM>& B, int g0) {
     const int tiles = (P.ny + TYR - 1) / TYR;
     const size_t lds = (size_t)NG * (TYR + 2) * NZ * sizeof(TE);
     static int resident = 0;
-    if (!resident) {
-        (void)hipFuncSetAttribute((const void*)k_eta_rows<PREC, NG, NZ, TE, TM>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        int per_cu = 0, ncu = 0, dev = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eta_rows<PREC, NG, NZ, TE, TM>, 512, lds);
-        resident = std::max(1, per_cu) * std::max(1, ncu);
-    }
+    if (!resident) resident = prepare_kernel(c, (const void*)k_eta_rows<PREC, NG, NZ, TE, TM>, 512, lds);
+    if (!resident) return;
     int nseg = pick_nseg(tiles, resident, P.nx, 1.);
     if (c->eta_nseg > 0) nseg = std::min(c->eta_nseg, P.nx);
     const int XS = (P.nx + nseg - 1) / nseg;
@@ -241,16 +260,8 @@ void launch_eta_march_ng(Ctx* c, const Bufs<TE, TM>& B, int g0) {
     const int tiles = nTz * nTy;
     const size_t lds = (size_t)NG * (TY + 2) * 66 * sizeof(TE);
     static int resident = 0;   // blocks of this instantiation that fit on the chip at once
-    if (!resident) {
-        (void)hipFuncSetAttribute((const void*)k_eta_march<PREC, NG, TY, TE, TM>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        int per_cu = 0, ncu = 0, dev = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eta_march<PREC, NG, TY, TE, TM>, 64 * TY,
-                                                           lds);
-        resident = std::max(1, per_cu) * std::max(1, ncu);
-    }
+    if (!resident) resident = prepare_kernel(c, (const void*)k_eta_march<PREC, NG, TY, TE, TM>, 64 * TY, lds);
+    if (!resident) return;
     // each segment loads one extra plane of every field at its start
     int nseg = pick_nseg(tiles, resident, P.nx, 1.);
     if (c->eta_nseg > 0) nseg = std::min(c->eta_nseg, P.nx);
@@ -312,15 +323,8 @@ int launch_pde_march_ty(Ctx* c, const Bufs<TE, TM>& B) {
     const int tiles = nTz * nTy;
     const size_t lds = PdeLds<PREC, TY>::bytes();
     static int resident = 0;
-    if (!resident) {
-        (void)hipFuncSetAttribute((const void*)k_pde_march<PREC, TY, TE, TM>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds);
-        int per_cu = 0, ncu = 0, dev = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_pde_march<PREC, TY, TE, TM>, 64 * TY, lds);
-        resident = std::max(1, per_cu) * std::max(1, ncu);
-    }
+    if (!resident) resident = prepare_kernel(c, (const void*)k_pde_march<PREC, TY, TE, TM>, 64 * TY, lds);
+    if (!resident) return -2;
     // the prologue of a segment costs about three planes (4 plane loads, one mu plane)
     int nseg = pick_nseg(tiles, resident, P.nx, 3.);
     if (c->pde_nseg > 0) nseg = std::min(c->pde_nseg, P.nx);
@@ -351,6 +355,7 @@ void launch_step(Ctx* c) {
     if constexpr (DIM == 3) {
         if (pde_march_applies(c)) nparts = launch_pde_march<PREC, TE, TM>(c, B);
     }
+    if (!c->launch_error.empty()) return;   // reported by nicr_step; nothing of this step is queued
     if (nparts < 0) {
         tmark_begin(c, T_MU);
         hipLaunchKernelGGL((k_mu<DIM, PREC, TE, TM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P, B);
@@ -363,6 +368,7 @@ void launch_step(Ctx* c) {
     if (c->eta_evolves()) {
         bool done = false;
         if constexpr (DIM == 3) done = launch_eta_march<PREC, TE, TM>(c, B);
+        if (!c->launch_error.empty()) return;
         if (!done) {
             tmark_begin(c, T_ETA);
             hipLaunchKernelGGL((k_eta<DIM, PREC, TE, TM>), dim3(c->g_eta), dim3(TILE), 0, c->stream, c->P, B);
@@ -422,6 +428,9 @@ void do_mask(Ctx* c) {
     }
 }
 void do_step(Ctx* c) {
+    // --frozen-eta-cache off (SL:346-349,599-603): a frozen eta mask is rebuilt
+    // by every step's refresh instead of only after an edit; same values
+    if (c->has_eta() && !c->eta_evolves() && !c->cfg.frozen_eta_cache) do_mask(c);
     if (c->prec == PREC_FP64) {
         if (c->dim == 2) launch_step<2, PREC_FP64, double, double>(c);
         else launch_step<3, PREC_FP64, double, double>(c);
@@ -888,8 +897,12 @@ int nicr_step(void* h, int nsteps, nicr_step_report* rep) {
         c->mask_dirty = false;
     }
     hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(1), 0, c->stream, c->ctl);
-    for (int s = 0; s < nsteps; ++s) do_step(c);
+    for (int s = 0; s < nsteps && c->launch_error.empty(); ++s) do_step(c);
     HIPCHK(hipGetLastError());
+    if (!c->launch_error.empty()) {
+        hipStreamSynchronize(c->stream);
+        return fail(NICR_E_DEVICE, "nicr_step: " + c->launch_error);
+    }
     DevCtl hc;
     int rc = sync_ctl(c, &hc);
     if (rc) return rc;
@@ -917,15 +930,18 @@ int nicr_step(void* h, int nsteps, nicr_step_report* rep) {
             r.j = (int)(yz / c->P.nz) + 1;
             r.k = c->dim == 3 ? (int)(yz % c->P.nz) + 1 : 0;
         } else {
+            // the key orders (field, cell) as the reference's loops do: g * ncell + cell;
+            // the field itself sits at g * fstride (padded) in the trial buffer
             const long long key = (long long)hc.eta_key;
             const long long g = key / c->ncell, q = key % c->ncell;
             r.status = NICR_STEP_ETA_RANGE;
             r.grain = (int)g + 1;
+            const char* src = (const char*)eta_ptr(c, nxt, (int)g) + (size_t)q * c->te;
             if (c->te == 8) {
-                HIPCHK(hipMemcpy(&r.eta_value, (double*)c->eta[nxt] + key, 8, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(&r.eta_value, src, 8, hipMemcpyDeviceToHost));
             } else {
                 float f;
-                HIPCHK(hipMemcpy(&f, (float*)c->eta[nxt] + key, 4, hipMemcpyDeviceToHost));
+                HIPCHK(hipMemcpy(&f, src, 4, hipMemcpyDeviceToHost));
                 r.eta_value = f;
             }
             r.i = (int)(q / nyz) + 1;
@@ -991,6 +1007,48 @@ int nicr_nucleate(void* h, int* inserted) {
     if (n && c->has_eta()) c->mask_dirty = true;
     c->refresh_valid = false;
     if (inserted) *inserted = n;
+    return NICR_OK;
+}
+
+int nicr_center_seed(void* h, int* cells) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c) return fail(NICR_E_INVALID, "null context");
+    if (cells) *cells = 0;
+    const double r = c->cfg.radius_grid;
+    const int nmin = c->dim == 3 ? std::min({c->P.nx, c->P.ny, c->P.nz}) : std::min(c->P.nx, c->P.ny);
+    if (r >= 0.5 * (double(nmin) - 1.))
+        return fail(NICR_E_INVALID, c->dim == 3 ? "Center seed is too large: --radius-grid must be less than (min(nx,ny,nz)-1)/2."
+                                                : "Center seed is too large: --radius-grid must be less than (min(nx,ny)-1)/2.");
+    DevCtl hc;
+    int rc = sync_ctl(c, &hc);
+    if (rc) return rc;
+    if (hc.nuclei != 0) return fail(NICR_E_STATE, "The center oxide has already been initialized.");
+    rc = ensure_nuc_scratch(c);
+    if (rc) return rc;
+    double* const* d = (double* const*)c->dev_obuf;
+    if (c->dim == 2) {
+        if (c->te == 8)
+            hipLaunchKernelGGL((k_center_seed<2, double>), dim3(1), dim3(1024), 0, c->stream, c->P, r, d, d + 2, d + 4,
+                               c->has_eta() ? (double*)c->eta[mask_index(c)] : nullptr, c->ctl, c->nuc_misc + 1);
+        else
+            hipLaunchKernelGGL((k_center_seed<2, float>), dim3(1), dim3(1024), 0, c->stream, c->P, r, d, d + 2, d + 4,
+                               c->has_eta() ? (float*)c->eta[mask_index(c)] : nullptr, c->ctl, c->nuc_misc + 1);
+    } else {
+        if (c->te == 8)
+            hipLaunchKernelGGL((k_center_seed<3, double>), dim3(1), dim3(1024), 0, c->stream, c->P, r, d, d + 2, d + 4,
+                               c->has_eta() ? (double*)c->eta[mask_index(c)] : nullptr, c->ctl, c->nuc_misc + 1);
+        else
+            hipLaunchKernelGGL((k_center_seed<3, float>), dim3(1), dim3(1024), 0, c->stream, c->P, r, d, d + 2, d + 4,
+                               c->has_eta() ? (float*)c->eta[mask_index(c)] : nullptr, c->ctl, c->nuc_misc + 1);
+    }
+    HIPCHK(hipGetLastError());
+    int n = 0;
+    HIPCHK(hipMemcpyAsync(&n, c->nuc_misc + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (n == 0) return fail(NICR_E_INVALID, "Center seed covers no cell centres: increase --radius-grid.");
+    if (c->has_eta()) c->mask_dirty = true;
+    c->refresh_valid = false;
+    if (cells) *cells = n;
     return NICR_OK;
 }
 

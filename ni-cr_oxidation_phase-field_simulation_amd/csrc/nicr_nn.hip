@@ -1589,6 +1589,18 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
     if (!n || !c || !opt || !rep || nsteps < 0) return fail(NICR_E_INVALID, "nicr_sur_advance: bad argument");
     int rc = check_ctx_shape(n, c);
     if (rc) return rc;
+    // A jump moves the state by a network proposal and, with conserve_mass,
+    // rescales O and Cr to their pre-jump totals. That is the model's own
+    // evolution only where the plain step adds no O or Cr from outside: no
+    // inlet (an open O boundary feeds bcO every step, 2D:408-411,435-442), no
+    // legacy Ck source (2D:567-571) and no Ksp scans falling due inside a jump
+    // (2D:1222). Any other model is refused rather than advanced wrongly.
+    if (c->cfg.oxygen_diffusion && c->cfg.bc != NICR_BC_CLOSED)
+        return fail(NICR_E_INVALID, "nicr_sur_advance: the model has an open O boundary (bc must be closed)");
+    if (c->cfg.ck == NICR_CK_LEGACY)
+        return fail(NICR_E_INVALID, "nicr_sur_advance: legacy Ck adds O and Cr every step (ck must be off)");
+    if (c->cfg.nucleation == NICR_NUC_KSP)
+        return fail(NICR_E_INVALID, "nicr_sur_advance: Ksp scans would fall inside jumps (nucleation must not be ksp)");
     const int J = opt->jump;
     if (J < 2) return fail(NICR_E_INVALID, "nicr_sur_advance: jump must be >= 2");
     if (J != n->norm.jump) return fail(NICR_E_INVALID, "nicr_sur_advance: jump differs from the network's");

@@ -335,4 +335,74 @@ __global__ __launch_bounds__(1024) void k_ksp_resolve(Par P, NucPar N, double* c
     }
 }
 
+// Centre seed (initialize_center_oxide, mpi2d/NiCr2D_grains.cpp:517-542,
+// mpi3d-slab/NiCr3D_grains.cpp:725-751): every cell centre with
+// (i-x)^2 + (j-y)^2 (+ (k-z)^2) <= r^2 about the box centre x = (nx+1)/2, ...,
+// no periodic image and no phi test, becomes hard oxide; eta is zeroed there.
+// One block walks the seed's bounding box; the inserted O and Cr go to the
+// nucO/nucCr budgets (double-double), the cell count to *cells_out. The caller
+// has checked the radius and that no nucleus exists yet; a seed covering no cell
+// changes nothing and records nothing.
+template <int DIM, class TE>
+__global__ __launch_bounds__(1024) void k_center_seed(Par P, double r, double* const* obuf, double* const* crbuf,
+                                                      double* const* pbuf, TE* eta, DevCtl* ctl, int* cells_out) {
+    __shared__ int sh_count;
+    __shared__ DD sh[16];
+    const int cur = ctl->cur;
+    double* o = obuf[cur];
+    double* cr = crbuf[cur];
+    double* p = pbuf[cur];
+    const double xc = 0.5 * ((double)P.nx + 1.), yc = 0.5 * ((double)P.ny + 1.), zc = 0.5 * ((double)P.nz + 1.);
+    // bounding box of the seed, clipped to the mesh (1-based)
+    const int i0 = max(1, (int)floor(xc - r)), i1 = min(P.nx, (int)ceil(xc + r));
+    const int j0 = max(1, (int)floor(yc - r)), j1 = min(P.ny, (int)ceil(yc + r));
+    const int k0 = DIM == 3 ? max(1, (int)floor(zc - r)) : 1, k1 = DIM == 3 ? min(P.nz, (int)ceil(zc + r)) : 1;
+    const long long cj = j1 - j0 + 1, ck = k1 - k0 + 1;
+    const long long nbox = (long long)(i1 - i0 + 1) * cj * ck;
+    if (threadIdx.x == 0) sh_count = 0;
+    __syncthreads();
+    DD aO{0., 0.}, aC{0., 0.};
+    int mine = 0;
+    for (long long q = threadIdx.x; q < nbox; q += blockDim.x) {
+        const int i = i0 + (int)(q / (cj * ck));
+        const long long rem = q % (cj * ck);
+        const int j = j0 + (int)(rem / ck), k = k0 + (int)(rem % ck);
+        const double dx = i - xc, dy = j - yc;
+        double d2 = dx * dx + dy * dy;
+        if (DIM == 3) {
+            const double dz = k - zc;
+            d2 = d2 + dz * dz;
+        }
+        if (d2 <= r * r) {
+            const long long c = (long long)(i - 1) * P.nyz + (long long)(j - 1) * P.nz + (DIM == 3 ? k - 1 : 0);
+            aO = dd_add(aO, Phys::oxideO - o[c]);
+            aC = dd_add(aC, Phys::oxideCr - cr[c]);
+            o[c] = Phys::oxideO;
+            cr[c] = Phys::oxideCr;
+            p[c] = 1.;
+            if (eta)
+                for (int g = 0; g < P.n_eta; ++g) eta[(long long)g * P.fstride + c] = (TE)0;
+            ++mine;
+        }
+    }
+    if (mine) atomicAdd(&sh_count, mine);
+    const DD tO = block_reduce_dd(aO, sh);
+    const DD tC = block_reduce_dd(aC, sh);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int n = sh_count;
+        if (n) {
+            DD b0{ctl->budget[0], ctl->budget[1]}, b1{ctl->budget[2], ctl->budget[3]};
+            b0 = dd_add(b0, tO);
+            b1 = dd_add(b1, tC);
+            ctl->budget[0] = b0.hi;
+            ctl->budget[1] = b0.lo;
+            ctl->budget[2] = b1.hi;
+            ctl->budget[3] = b1.lo;
+            ctl->nuclei += 1;
+        }
+        *cells_out = n;
+    }
+}
+
 }  // namespace nicr

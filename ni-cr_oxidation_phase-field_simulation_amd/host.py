@@ -37,11 +37,23 @@ def host_library():
         lib.nicr_host_budget.argtypes = [ctypes.c_void_p, ctypes.POINTER(NicrBudget), _DPTR]
         lib.nicr_host_set_budget.argtypes = [ctypes.c_void_p, ctypes.POINTER(NicrBudget)]
         lib.nicr_host_seeds.argtypes = [ctypes.c_void_p, _DPTR]
+        lib.nicr_host_uniform_noise.argtypes = [ctypes.c_ulonglong, ctypes.c_size_t, ctypes.c_double,
+                                                ctypes.c_double, _DPTR]
         lib.nicr_host_write.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p]
         lib.nicr_host_parse.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
                                         ctypes.POINTER(NicrConfig), ctypes.c_char_p, ctypes.c_size_t]
         _HLIB = lib
     return _HLIB
+
+
+def uniform_noise(shape, lo: float, hi: float, seed: int = 20260909) -> np.ndarray:
+    """std::uniform_real_distribution<double>(lo, hi) over std::mt19937_64(seed),
+    drawn in the reference's loop order (i, then j, then k = storage order)."""
+    lib = host_library()
+    out = np.empty(shape, dtype=np.float64)
+    if lib.nicr_host_uniform_noise(int(seed), out.size, float(lo), float(hi), out.ctypes.data_as(_DPTR)) != 0:
+        raise RuntimeError(lib.nicr_host_last_error().decode())
+    return out
 
 
 OPTION_KEYS = ("bc", "ck", "bounds", "nucleation", "out", "eta_output", "output_format", "output_precision",

@@ -42,10 +42,17 @@ def oxide_particle(shape, centre=None, radius=4.0, width=1.5):
 
 
 def noisy_alloy(shape, cr_mean=0.20, cr_noise=0.005, seed=20260909, layer=True):
-    """O = 0 in the bulk, Cr = cr_mean + U(-noise, noise), inlet layer at i = 1."""
-    rng = np.random.default_rng(seed)
+    """O = 0 in the bulk, Cr = cr_mean + U(-noise, noise), inlet layer at i = 1.
+
+    The noise is std::uniform_real_distribution<double> over
+    std::mt19937_64(seed) drawn in the reference's loop order (the generator
+    the reference seeds its grains with, mpi3d-slab/NiCr3D_grains.cpp:425-427),
+    taken from the host library so that C++ drivers rebuild the same state.
+    The concentration-BC layer then overwrites plane i = 1 as the reference's
+    constructor does (mpi2d/NiCr2D_grains.cpp:214-219)."""
+    from .host import uniform_noise
     o = np.zeros(shape)
-    cr = cr_mean + rng.uniform(-cr_noise, cr_noise, size=shape)
+    cr = cr_mean + uniform_noise(shape, -cr_noise, cr_noise, seed)
     p = np.zeros(shape)
     if layer:
         o[0] = LAYER_O

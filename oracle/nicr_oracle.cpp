@@ -309,6 +309,25 @@ public:
         o[q]=Phys::oxideO;cr[q]=Phys::oxideCr;p[q]=1.;++changed;
         for(auto& e:eta)e[q]=0.;
     }
+    // initialize_center_oxide, 2D:517-542 / SL:725-751: returns the cell count,
+    // -1 for a radius at or above (min side - 1)/2, -2 with a nucleus already
+    // recorded (the reference throws); 0 cells leave nuclei unchanged.
+    int center_seed(){
+        const double x=0.5*(double(nx)+1.),y=0.5*(double(ny)+1.),z=0.5*(double(nz)+1.);
+        const double r=c.radius_grid;
+        const int nmin=dim==2?std::min(nx,ny):std::min({nx,ny,nz});
+        if(r>=0.5*(double(nmin)-1.))return -1;
+        if(b.nuclei!=0)return -2;
+        int cells=0;
+        for(int i=1;i<=nx;++i)for(int j=1;j<=ny;++j)for(int k=klo();k<=khi();++k){
+            const double dx=i-x,dy=j-y;
+            double d2=dx*dx+dy*dy;
+            if(dim==3){const double dz=k-z;d2=dx*dx+dy*dy+dz*dz;}
+            if(d2<=r*r)stamp_cell(ix(i,j,k),cells);
+        }
+        if(cells)++b.nuclei;
+        return cells;
+    }
     // 2D:503-516 / SL:704-724
     int stamp(int x,int y,int z){
         int changed=0;
@@ -422,6 +441,16 @@ int ora_step(void* h,int nsteps,nicr_step_report* rep){
     return NICR_OK;
 }
 int ora_nucleate(void* h,int* cells){int n=static_cast<Oracle*>(h)->nucleate();if(cells)*cells=n;return NICR_OK;}
+int ora_center_seed(void* h,int* cells){
+    Oracle* m=static_cast<Oracle*>(h);
+    const int n=m->center_seed();
+    if(cells)*cells=n>0?n:0;
+    if(n==-1){g_error=m->dim==3?"Center seed is too large: --radius-grid must be less than (min(nx,ny,nz)-1)/2."
+                                :"Center seed is too large: --radius-grid must be less than (min(nx,ny)-1)/2.";return NICR_E_INVALID;}
+    if(n==-2){g_error="The center oxide has already been initialized.";return NICR_E_STATE;}
+    if(n==0){g_error="Center seed covers no cell centres: increase --radius-grid.";return NICR_E_INVALID;}
+    return NICR_OK;
+}
 int ora_stamp(void* h,int x,int y,int z,int* changed){int n=static_cast<Oracle*>(h)->stamp(x,y,z);if(changed)*changed=n;return NICR_OK;}
 // Model::sum in long double, returned split as hi/lo binary64.
 int ora_sum(void* h,int id,double out[2]){

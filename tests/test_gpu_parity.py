@@ -140,21 +140,50 @@ def test_stop_bounds_rejection(nicr, oracle, dim):
         assert np.array_equal(gpu.get("o"), o_before)   # inlet restored (SL:815-818)
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_eta_range_rejection(nicr, oracle, dim):
-    """An eta trial outside [0,1] rejects every field (2D:351-355, 2D:1150-1157)."""
-    shape = (21, 9) if dim == 2 else (15, 9, 7)
-    gpu, ora = _pair(nicr, oracle, dim, shape, 2, True, bc="closed", bounds="legacy", ck="off")
-    e = gpu.get(0)
+ETA_REJECT_CASES = [
+    # dim, shape, n_eta, failing field (0-based); the 3D shapes cover the
+    # unblocked kernel (nz 7), the whole-row marching kernel (nz 64, passes of
+    # 8 fields in binary64 and 16 in mixed precision) and 64-wide tiles (nz 40)
+    (2, (21, 9), 2, 0),
+    (2, (21, 9), 4, 2),
+    (3, (15, 9, 7), 2, 0),
+    (3, (15, 9, 7), 5, 3),
+    (3, (6, 5, 64), 20, 17),
+    (3, (5, 6, 40), 12, 9),
+]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", range(len(ETA_REJECT_CASES)))
+def test_eta_range_rejection(nicr, oracle, case, prec):
+    """An eta trial outside [0,1] rejects every field (2D:351-355, SL:518-522).
+
+    The report names the first failing (field, cell) in the reference's loop
+    order (field outer, then i, j, k) and the trial value stored there. A second
+    bad value in the next field at an earlier cell must not win."""
+    dim, shape, n_eta, gfail = ETA_REJECT_CASES[case]
+    gpu, ora = _pair(nicr, oracle, dim, shape, n_eta, True, bc="closed", bounds="legacy", ck="off",
+                     precision=prec)
     idx = (shape[0] - 1, 4) if dim == 2 else (shape[0] - 1, 3, 2)
-    e[idx] = 2.0
-    gpu.set(0, e)
-    ora.set(0, e)
+    early = (1, 1) if dim == 2 else (1, 1, 1)
+    for g, where, val in ((gfail, idx, 2.0), (gfail + 1, early, -1.5)):
+        e = gpu.get(g)
+        e[where] = val
+        gpu.set(g, e)
+        ora.set(g, e)
     rg = gpu.try_step(2)
     ro = ora.try_step(2)
     assert rg.status == ro.status == nicr.STEP_ETA_RANGE
-    assert nicr.step_error_message(rg, dim) == nicr.step_error_message(ro, dim)
-    _compare(gpu, ora, "after eta rejection", refresh=False)
+    assert rg.grain == ro.grain == gfail + 1
+    assert (rg.i, rg.j, rg.k) == (ro.i, ro.j, ro.k) == tuple(v + 1 for v in idx) + ((0,) if dim == 2 else ())
+    if prec == "fp64":
+        assert nicr.step_error_message(rg, dim) == nicr.step_error_message(ro, dim)
+        _compare(gpu, ora, "after eta rejection", refresh=False)
+    else:
+        # binary32 eta: the stored trial value to binary32 rounding of the force
+        assert abs(rg.eta_value - ro.eta_value) <= 1e-6 * abs(ro.eta_value)
+        for f in ("o", "cr", "p"):
+            assert np.array_equal(gpu.get(f), ora.get(f)), f"mixed rejection changed {f}"
 
 
 @pytest.mark.parametrize("dim", [2, 3])
@@ -263,3 +292,47 @@ def test_stop_bounds_rejection_marching_kernel(nicr, oracle, dim, shape):
     assert nicr.step_error_message(rg, dim) == nicr.step_error_message(ro, dim)
     _compare(gpu, ora, "after rejection", refresh=False)
     assert np.array_equal(gpu.get("o"), o_before)
+
+
+@pytest.mark.parametrize("shape", [(23, 13, 11), (6, 5, 64)])
+def test_frozen_eta_cache_off_equals_on(nicr, oracle, shape):
+    """--frozen-eta-cache off rebuilds the frozen mask every step and must give
+    the bitwise same run as the cached mask (SL:346-349,599-603;
+    mpi3d-slab/tests/regression_driver.cpp:60-64), including after a stamp that
+    invalidates the cache (SL:722)."""
+    runs = []
+    for cache in (1, 0):
+        gpu, ora = _pair(nicr, oracle, 3, shape, 3, True, bc="concentration", bounds="legacy", eta_L=0.0,
+                         frozen_eta_cache=cache)
+        for m in (gpu, ora):
+            m.step(4)
+            m.stamp(shape[0] // 2, 2, 3)
+            m.step(4)
+        _compare(gpu, ora, f"cache={'on' if cache else 'off'}")
+        runs.append([gpu.get(f) for f in ("o", "cr", "p", "gb", "eta_s2", 0, 1, 2)] + [gpu.b])
+    for a, b in zip(runs[0][:-1], runs[1][:-1]):
+        assert np.array_equal(a, b)
+    assert runs[0][-1] == runs[1][-1]
+
+
+def test_2d_eta_trial_runs_with_zero_mobility(nicr, oracle):
+    """2D runs the eta trial even when L_eta = 0 (2D:592), so an eta value outside
+    [0,1] still rejects the step; 3D skips the trial and the swap (SL:810,822)."""
+    for dim, shape in ((2, (21, 17)), (3, (15, 9, 7))):
+        gpu, ora = _pair(nicr, oracle, dim, shape, 3, True, bc="closed", bounds="legacy", ck="off", eta_L=0.0)
+        for m in (gpu, ora):
+            m.step(3)
+        _compare(gpu, ora, f"dim {dim} L_eta=0")
+        idx = (4, 5) if dim == 2 else (4, 5, 3)
+        e = gpu.get(1)
+        e[idx] = 1.5
+        gpu.set(1, e)
+        ora.set(1, e)
+        rg, ro = gpu.try_step(2), ora.try_step(2)
+        assert rg.status == ro.status
+        if dim == 2:
+            assert rg.status == nicr.STEP_ETA_RANGE and rg.grain == 2
+            assert nicr.step_error_message(rg, dim) == nicr.step_error_message(ro, dim)
+        else:
+            assert rg.status == nicr.STEP_OK and rg.steps_done == 2
+        _compare(gpu, ora, f"dim {dim} after the edited eta", refresh=False)

@@ -197,6 +197,28 @@ def _plain_norm(net, J):
                  gain=[0.05, 0.05, 0.0], jump=J)
 
 
+
+@pytest.mark.parametrize("change", [dict(bc="concentration"), dict(bc="mu"), dict(ck="legacy"),
+                                    dict(nucleation="ksp")], ids=["inlet", "mu-inlet", "ck-legacy", "ksp"])
+def test_advance_refuses_models_it_cannot_jump(nicr, change):
+    """A jump is the model's own evolution only without an O inlet, legacy Ck
+    or Ksp scans; any such model is refused with NICR_E_INVALID and left
+    untouched."""
+    shape = (16, 16, 24)
+    state = _syn().closed_alloy(shape, n_seeds=8, seed=4)
+    kw = dict(grains=True, bc="closed", ck="off", bounds="legacy", nucleation="off")
+    kw.update(change)
+    cfg = nicr.make_config(3, nx=shape[0], ny=shape[1], nz=shape[2], **kw)
+    m = nicr.Model(cfg)
+    o, cr, p, gb = state
+    m.set("o", o); m.set("cr", cr); m.set("p", p); m.set("gb", gb)
+    net = _sur().UNet(shape, seed=1)
+    _plain_norm(net, 20)
+    with pytest.raises(RuntimeError, match="nicr_sur_advance"):
+        net.advance(m, 60, jump=20, residual_tol=0.3)
+    assert np.array_equal(m.get("o"), o) and np.array_equal(m.get("cr"), cr)
+    net.close()
+
 @pytest.mark.parametrize("n_eta,settle,fallback", [(0, 0, 0), (8, 0, 0), (8, 2, 7)],
                          ids=["static", "evolving", "evolving-settle2-fallback7"])
 def test_rejected_jumps_equal_plain_steps(nicr, n_eta, settle, fallback):

@@ -147,3 +147,43 @@ def test_sum_long_double(nicr, oracle, dim):
     ora.try_step(3)
     for f in ("o", "cr", "p"):
         assert ref.sum(f) == ora.sum(f)
+
+
+SEED_CASES = [
+    (2, dict(nx=40, ny=33, grains=False, radius_grid=5.0)),
+    (2, dict(nx=61, ny=60, grain_diameter_grid=15., radius_grid=5.5, bc="closed")),
+    (3, dict(nx=20, ny=18, nz=16, grains=False, radius_grid=4.0)),
+    (3, dict(nx=24, ny=21, nz=16, grain_diameter_grid=9., radius_grid=3.5, bc="mu")),
+]
+
+
+@pytest.mark.parametrize("case", range(len(SEED_CASES)))
+def test_center_seed_bitwise(nicr, oracle, case):
+    """The oracle's centre seed on the reference's pre-seed state equals the
+    reference constructor's seeded state: fields, nucO/nucCr, nuclei and the cell
+    count (initialize_center_oxide, 2D:517-542, SL:725-751)."""
+    dim, kw = SEED_CASES[case]
+    if not oracle.ref_available(dim):
+        pytest.skip("reference sources not built into oracle/_ref")
+    cfg_off = nicr.make_config(dim, nucleation="off", **kw)
+    cfg_on = nicr.make_config(dim, nucleation="center", **kw)
+    ref_off, ref_on = oracle.RefModel(cfg_off), oracle.RefModel(cfg_on)
+    ora = oracle.OracleModel(cfg_off)
+    oracle.copy_state(ref_off, ora)
+    cells = ora.initialize_center_oxide()
+    assert cells == ref_on.center_seed_cells() > 0
+    for f in ("o", "cr", "p"):
+        assert np.array_equal(ora.get(f), ref_on.get(f)), f
+    rb, ob = ref_on.budget_raw(), ora.budget_raw()
+    for k in ("nucO", "nucCr"):
+        assert tuple(getattr(rb, k)) == tuple(getattr(ob, k)), k
+    assert rb.nuclei == ob.nuclei == 1
+    with pytest.raises(RuntimeError, match="already been initialized"):
+        ora.initialize_center_oxide()
+
+
+def test_center_seed_radius_refused(nicr, oracle):
+    cfg = nicr.make_config(2, nx=12, ny=12, grains=False, nucleation="off", radius_grid=5.5)
+    ora = oracle.OracleModel(cfg)
+    with pytest.raises(RuntimeError, match="Center seed is too large"):
+        ora.initialize_center_oxide()
