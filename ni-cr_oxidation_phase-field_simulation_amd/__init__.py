@@ -304,6 +304,40 @@ class Model:
         self._h.check(self._h._fn("step")(self._h.h, int(nsteps), ctypes.byref(rep)))
         return rep
 
+    def run_loop(self, nsteps: int, first: int = 1, next_event: float | None = None) -> dict:
+        """Steps first .. first + nsteps - 1 of the reference run loop without
+        output (mpi2d/NiCr2D_grains.cpp:1216-1223, mpi3d-slab/NiCr3D_grains.cpp:1365-1372):
+        with Ksp nucleation, Model::nucleate runs before step n whenever
+        n dt + 1e-12 period >= the next event time, which then advances by one
+        period. The steps between events are queued as one nicr_step call.
+        Returns {"steps", "scans", "inserted", "next_event", "report"}; stops at a
+        rejected step like the reference (report.status != 0)."""
+        c = self.c
+        period = c.nucleation_period
+        nxt = period if next_event is None else next_event
+        ksp = c.nucleation == NUCLEATION["ksp"]
+        done = scans = inserted = 0
+        n = first
+        last = first + nsteps - 1
+        rep = NicrStepReport()
+        while n <= last:
+            if ksp and n * c.dt + 1.e-12 * period >= nxt:
+                inserted += self.nucleate()
+                scans += 1
+                nxt += period
+            # the steps before the next event (event k fires before step n_k)
+            m = n
+            while ksp and m < last and (m + 1) * c.dt + 1.e-12 * period < nxt:
+                m += 1
+            if not ksp:
+                m = last
+            rep = self.try_step(m - n + 1)
+            done += rep.steps_done
+            if rep.status != STEP_OK:
+                break
+            n = m + 1
+        return {"steps": done, "scans": scans, "inserted": inserted, "next_event": nxt, "report": rep}
+
     # ---- insertion ----
     def nucleate(self) -> int:
         n = ctypes.c_int(0)

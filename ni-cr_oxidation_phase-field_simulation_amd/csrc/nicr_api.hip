@@ -62,6 +62,7 @@ struct Ctx {
     void* s2[2] = {nullptr, nullptr};
     void* pairs[2] = {nullptr, nullptr};
     void* gb[2] = {nullptr, nullptr};
+    float* ecoef = nullptr;     // eta-force factor per cell (mixed, marching kernels; see Bufs)
     double* partials = nullptr;
     DevCtl* ctl = nullptr;
     double* stat_part = nullptr;
@@ -83,6 +84,7 @@ struct Ctx {
     int pde_nseg = 0;              // X segments of the marching PDE kernel (NICR_PDE_NSEG; 0: auto)
     int cur = 0;                 // host mirror of ctl->cur
     bool mask_dirty = true;
+    bool pairs_stale = false;      // the last eta pass of a step leaves P unwritten
     bool refresh_valid = false;
     bool march_disabled = false;   // NICR_KERNEL_VARIANT=basic selects the unblocked kernels
     int eta_ty = 8;                // Y tile height of the marching eta kernel (NICR_ETA_TY: 2, 4, 8)
@@ -124,6 +126,7 @@ Bufs<TE, TM> bufs(Ctx* c) {
     }
     b.mu_o = (TM*)c->mu_o;
     b.mu_cr = (TM*)c->mu_cr;
+    b.ecoef = c->ecoef;
     b.partials = c->partials;
     b.ctl = c->ctl;
     return b;
@@ -161,9 +164,9 @@ void tcollect(Ctx* c) {
 
 // ---- launches, dispatched over (DIM, PREC) ----
 template <int DIM, int PREC, class TE, class TM>
-void launch_mask(Ctx* c) {
+void launch_mask(Ctx* c, int pairs_only) {
     hipLaunchKernelGGL((k_mask<DIM, PREC, TE, TM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P,
-                       bufs<TE, TM>(c), c->eta_evolves() ? -1 : 0);
+                       bufs<TE, TM>(c), c->eta_evolves() ? -1 : 0, pairs_only);
 }
 // X segments per column tile for a marching kernel: every block marches nx/nseg
 // planes after a prologue of about `lead` planes, and the blocks run in
@@ -357,6 +360,7 @@ void launch_step(Ctx* c) {
     }
     if (!c->launch_error.empty()) return;   // reported by nicr_step; nothing of this step is queued
     if (nparts < 0) {
+        B.ecoef = nullptr;   // written only by the marching PDE kernel
         tmark_begin(c, T_MU);
         hipLaunchKernelGGL((k_mu<DIM, PREC, TE, TM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P, B);
         tmark_end(c);
@@ -417,15 +421,16 @@ void launch_nucleate(Ctx* c, NucPar N) {
                        c->nuc_centres, c->nuc_misc + 1);
 }
 
-void do_mask(Ctx* c) {
+void do_mask(Ctx* c, int pairs_only = 0) {
     if (!c->has_eta()) return;
     if (c->prec == PREC_FP64) {
-        if (c->dim == 2) launch_mask<2, PREC_FP64, double, double>(c);
-        else launch_mask<3, PREC_FP64, double, double>(c);
+        if (c->dim == 2) launch_mask<2, PREC_FP64, double, double>(c, pairs_only);
+        else launch_mask<3, PREC_FP64, double, double>(c, pairs_only);
     } else {
-        if (c->dim == 2) launch_mask<2, PREC_MIXED, float, float>(c);
-        else launch_mask<3, PREC_MIXED, float, float>(c);
+        if (c->dim == 2) launch_mask<2, PREC_MIXED, float, float>(c, pairs_only);
+        else launch_mask<3, PREC_MIXED, float, float>(c, pairs_only);
     }
+    c->pairs_stale = false;
 }
 void do_step(Ctx* c) {
     // --frozen-eta-cache off (SL:346-349,599-603): a frozen eta mask is rebuilt
@@ -604,6 +609,8 @@ int nicr_create(const nicr_config* cfg, void** out) {
     P.dy = c->cfg.dy;
     P.dz = c->cfg.dz;
     P.dt = c->cfg.dt;
+    P.ck_o_dt = Phys::ckO / P.dt;
+    P.ck_cr_dt = Phys::ckCr / P.dt;
     P.gbfo = c->cfg.gb_factor_o;
     P.gbfc = c->cfg.gb_factor_cr;
     P.mu_res = c->cfg.mu_res;
@@ -690,6 +697,9 @@ int nicr_create(const nicr_config* cfg, void** out) {
     c->g_stat = 512;
     c->partial_slots = std::max(c->g_trial, 8192);
     if (rc == NICR_OK) rc = dalloc(c, (void**)&c->partials, (size_t)c->partial_slots * NPART * 8);
+    const char* ec = std::getenv("NICR_ETA_COEF");
+    if (rc == NICR_OK && c->prec == PREC_MIXED && c->eta_evolves() && pde_march_applies(c) && !(ec && ec[0] == '0'))
+        rc = dalloc(c, (void**)&c->ecoef, (size_t)c->ncell * sizeof(float));
     if (rc == NICR_OK) rc = dalloc(c, (void**)&c->ctl, sizeof(DevCtl));
     if (rc == NICR_OK) rc = dalloc(c, (void**)&c->stat_part, (size_t)c->g_stat * 4 * 8);
     if (rc == NICR_OK) rc = dalloc(c, (void**)&c->dev_obuf, 6 * sizeof(double*));
@@ -819,6 +829,7 @@ int nicr_download_field(void* h, int field, double* dstp) {
         do_mask(c);
         c->mask_dirty = false;
     }
+    if (field == NICR_FIELD_ETA_PAIRS && c->pairs_stale) do_mask(c, 1);
     if (convert) {
         int rc = pinned_stage(c);
         if (rc) return rc;
@@ -898,6 +909,7 @@ int nicr_step(void* h, int nsteps, nicr_step_report* rep) {
     }
     hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(1), 0, c->stream, c->ctl);
     for (int s = 0; s < nsteps && c->launch_error.empty(); ++s) do_step(c);
+    if (nsteps > 0 && c->eta_evolves()) c->pairs_stale = true;
     HIPCHK(hipGetLastError());
     if (!c->launch_error.empty()) {
         hipStreamSynchronize(c->stream);

@@ -444,6 +444,8 @@ int nicr_batch_create(const nicr_config* common, const nicr_batch_instance* in, 
     P.dy = common->dy;
     P.dz = common->dz;
     P.dt = common->dt;
+    P.ck_o_dt = Phys::ckO / P.dt;
+    P.ck_cr_dt = Phys::ckCr / P.dt;
     P.gbfo = common->gb_factor_o;
     P.gbfc = common->gb_factor_cr;
     P.mu_res = common->mu_res;

@@ -79,6 +79,7 @@ struct Par {
     double gbfo, gbfc, mu_res, c_ref;
     double etaW, etaK, etaL, etaA, eta_scale;
     float fdx2i, fdy2i, fdz2i, fdxi, fdyi, fdzi;   // binary32 reciprocals for PREC_MIXED
+    double ck_o_dt, ck_cr_dt;     // Ck_O / dt, Ck_Cr / dt (PREC_MIXED)
 };
 
 // Buffers. TE is the eta/mask storage type (double, or float for PREC_MIXED);
@@ -94,6 +95,11 @@ struct Bufs {
     TE* s2[2];
     TE* pairs[2];
     TE* gb[2];      // static mask: gb[0] == gb[1]
+    // PREC_MIXED with the marching kernels: the per-cell factor of the eta force,
+    // W_eta (S - (1 - h)) + 2 W_eta S + A_eta h of the step's input state,
+    // written by the PDE kernel (which holds S and phi of every cell) for the
+    // eta passes; nullptr: the eta kernels form it from phi and S themselves
+    float* ecoef;
     double* partials;
     DevCtl* ctl;
 };

@@ -17,8 +17,11 @@ __device__ __forceinline__ bool tile_cell(const Par& P, int tile, int& x, int& y
 // k_mask: S = sum eta^2, P = sum_{a<b} eta_a^2 eta_b^2 (running form of
 // 2D:332), gb = scale * min(1, 16 P)  (rebuild_eta_mask, 2D:328-340, SL:495-507)
 // ---------------------------------------------------------------------------
+// pairs_only: write P alone (the step's last eta pass leaves P unwritten, as
+// nothing in the step reads it; downloads rebuild it here without touching the
+// S and gb that the next step reads).
 template <int DIM, int PREC, class TE, class TM>
-__global__ __launch_bounds__(TILE) void k_mask(Par P, Bufs<TE, TM> B, int which) {
+__global__ __launch_bounds__(TILE) void k_mask(Par P, Bufs<TE, TM> B, int which, int pairs_only) {
     int x, y, z;
     const int t = xcd_remap(blockIdx.x, P.ntiles);
     if (!tile_cell(P, t, x, y, z)) return;
@@ -34,8 +37,9 @@ __global__ __launch_bounds__(TILE) void k_mask(Par P, Bufs<TE, TM> B, int which)
             S += q;
         }
         const double s16 = 16. * pr;
-        B.s2[e][c] = (TE)S;
         B.pairs[e][c] = (TE)pr;
+        if (pairs_only) return;
+        B.s2[e][c] = (TE)S;
         B.gb[e][c] = (TE)(P.eta_scale * (s16 < 1. ? s16 : 1.));
     } else {
         float S = 0.f, pr = 0.f;
@@ -46,8 +50,9 @@ __global__ __launch_bounds__(TILE) void k_mask(Par P, Bufs<TE, TM> B, int which)
             S += q;
         }
         const float s16 = 16.f * pr;
-        B.s2[e][c] = (TE)S;
         B.pairs[e][c] = (TE)pr;
+        if (pairs_only) return;
+        B.s2[e][c] = (TE)S;
         B.gb[e][c] = (TE)((float)P.eta_scale * (s16 < 1.f ? s16 : 1.f));
     }
 }
@@ -113,6 +118,15 @@ __device__ __forceinline__ void mu32_vals(const Par& P, const double* o7, const 
         - float(2. * Phys::ko) * lapo;
     muCr = (1.f - hh) * (1166528.60789006f * fc + 3200000.f * fo) + hh * (14000000.f * fc - 13800000.f * fo + 2777045.64f)
          - float(2. * Phys::kc) * lapc;
+}
+
+// The per-cell factor of the binary32 eta force, F_g = eta_g (coef - 2 W_eta
+// eta_g^2) - K lap(eta_g), with coef = W_eta (S - (1 - h)) + 2 W_eta S + A_eta h
+// (grain_local_deta, 2D:100-101, SL:108-109).
+__device__ __forceinline__ float eta_coef32(const Par& P, float S, float pphi) {
+#pragma clang fp contract(fast)
+    const float hphi = h32(pphi);
+    return (float)P.etaW * (S - (1.f - hphi)) + 2.f * (float)P.etaW * S + (float)P.etaA * hphi;
 }
 
 // o, cr at the 7 points of a cell from global memory, with the inlet substitution.
@@ -360,12 +374,28 @@ struct StepAcc {
 // (2D:574-580) or rejection (2D:581-589), inlet budget (2D:408-411,553) with
 // inlet O layer_o. binary64 in both precisions. Writes the trial values (raw
 // at a failing cell).
+// binary64 h(phi) with fused multiply-adds (mixed precision only)
+__device__ __forceinline__ double h64_fma(double p) {
+#pragma clang fp contract(fast)
+    return p * p * p * (10. + p * (-15. + 6. * p));
+}
+
+template <int PREC = PREC_FP64>
 __device__ __forceinline__ void finish_cell(const Par& P, long long c, int x, double o_stored, double oo, double cc,
                                             double pp, double onv, double crv, double pnv, double left_flux,
                                             StepAcc& a, double* on, double* crn, double* pn, double layer_o) {
     if (P.ck_legacy) {
-        double dO = -Phys::ckO * ((Phys::oxideO - onv) * h64(pnv) - (Phys::oxideO - oo) * h64(pp)) / P.dt;
-        double dCr = -Phys::ckCr * ((Phys::oxideCr - crv) * h64(pnv) - (Phys::oxideCr - cc) * h64(pp)) / P.dt;
+        double dO, dCr;
+        if constexpr (PREC == PREC_FP64) {
+            dO = -Phys::ckO * ((Phys::oxideO - onv) * h64(pnv) - (Phys::oxideO - oo) * h64(pp)) / P.dt;
+            dCr = -Phys::ckCr * ((Phys::oxideCr - crv) * h64(pnv) - (Phys::oxideCr - cc) * h64(pp)) / P.dt;
+        } else {
+            // mixed: still binary64, with Ck/dt as one factor; each product is
+            // rounded on its own, so an unchanged cell gives exactly zero
+            const double hn = h64_fma(pnv), ho = h64_fma(pp);
+            dO = -P.ck_o_dt * (__dmul_rn(Phys::oxideO - onv, hn) - __dmul_rn(Phys::oxideO - oo, ho));
+            dCr = -P.ck_cr_dt * (__dmul_rn(Phys::oxideCr - crv, hn) - __dmul_rn(Phys::oxideCr - cc, ho));
+        }
         onv += dO;
         crv += dCr;
         // adding a zero leaves a double-double sum bitwise unchanged (its parts
@@ -464,7 +494,7 @@ __global__ __launch_bounds__(TILE) void k_trial(Par P, Bufs<TE, TM> B) {
         const double S = P.has_eta ? (double)s2[c] : 0.;
         double onv, crv, pnv, left_flux;
         trial_mm<DIM, PREC, T>(P, m, p7, o7[SC], c7[SC], x, S, (double)gb[c], m64O, onv, crv, pnv, left_flux);
-        finish_cell(P, c, x, o[c], o7[SC], c7[SC], p7[SC], onv, crv, pnv, left_flux, acc, B.o[nxt], B.cr[nxt],
+        finish_cell<PREC>(P, c, x, o[c], o7[SC], c7[SC], p7[SC], onv, crv, pnv, left_flux, acc, B.o[nxt], B.cr[nxt],
                     B.p[nxt], P.layer_o);
     }
     write_partials(acc, B.partials, ctl);
@@ -663,13 +693,18 @@ __global__ __launch_bounds__(64 * TY) void k_pde_march(Par P, Bufs<TE, TM> B, in
     int so_x = 1, so_x1 = 2, so_x2 = 0;   // o, cr slots of planes x, x+1, x+2
     int sp_x = 0, sp_x1 = 1, sp_x2 = 2;   // phi slots of planes x, x+1 and of the next plane loaded
     StepAcc acc;
+    // S of the own cell, one plane ahead: its load is issued before the plane
+    // prefetch, so using it never waits for the prefetch (loads retire in order)
+    const bool sload = active && P.has_eta;
+    TE s_next = sload ? s2[(long long)x0 * plane + yz] : (TE)0;
     for (int x = x0; x < x1; ++x) {
         const int m0 = (x - x0) & 1, m1 = m0 ^ 1;   // gb, mu/M slots of planes x (and x+2), x+1
+        const double S = (double)s_next;
+        if (sload && x + 1 < x1) s_next = s2[(long long)(x + 1) * plane + yz];
         // A: planes x+3 (o, cr) and x+2 (phi, gb) into registers
         fetch_oc(x + 3);
         fetch_pg(x + 2);
         const long long c = (long long)x * plane + yz;
-        const double S = active && P.has_eta ? (double)s2[c] : 0.;
         // B: mu, M of plane x+1; o, cr of the own cell of plane x
         mm_plane(so_x, so_x1, so_x2, sp_x1, m1, m1);
         const double oo = so[so_x * P2 + own2], cc = sc[so_x * P2 + own2];
@@ -716,12 +751,15 @@ __global__ __launch_bounds__(64 * TY) void k_pde_march(Par P, Bufs<TE, TM> B, in
         p7[SZP] = sp[sp_x * P1 + own1 + 1];
         p7[SXP] = sp[sp_x1 * P1 + own1];
         p7[SXM] = p_prev;
+        if constexpr (PREC != PREC_FP64) {
+            if (active && B.ecoef) B.ecoef[c] = eta_coef32(P, (float)S, (float)p7[SC]);
+        }
         if (active) {
             const double g_c = x == 0 ? (double)gb[c] : 0.;
             const double o_stored = x == 0 ? o[c] : oo;
             double onv, crv, pnv, left_flux;
             trial_mm<3, PREC, T>(P, m, p7, oo, cc, x, S, g_c, m64O, onv, crv, pnv, left_flux);
-            finish_cell(P, c, x, o_stored, oo, cc, p7[SC], onv, crv, pnv, left_flux, acc, B.o[nxt], B.cr[nxt],
+            finish_cell<PREC>(P, c, x, o_stored, oo, cc, p7[SC], onv, crv, pnv, left_flux, acc, B.o[nxt], B.cr[nxt],
                         B.p[nxt], P.layer_o);
         }
         m_prev = m[SC];
@@ -740,6 +778,52 @@ __global__ __launch_bounds__(64 * TY) void k_pde_march(Par P, Bufs<TE, TM> B, in
         sp_x2 = u;
     }
     write_partials(acc, B.partials, ctl);
+}
+
+// ---------------------------------------------------------------------------
+// k_commit: all-or-nothing acceptance (collective_error + swap, 2D:594-596;
+// budget restore on rejection, SL:812-820). One block, after every other
+// kernel of the step.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void commit_body(const double* partials, DevCtl* ctl, int nparts) {
+    const unsigned long long pk = __hip_atomic_load(&ctl->pde_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long ek = __hip_atomic_load(&ctl->eta_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (pk != NO_KEY || ek != NO_KEY) {
+        if (threadIdx.x == 0) {
+            ctl->failed = 1;
+            ctl->fail_kind = pk != NO_KEY ? 1 : 2;
+        }
+        return;
+    }
+    __shared__ DD sh[16];
+    DD acc[6];
+    for (int s = 0; s < 6; ++s) acc[s] = DD{0., 0.};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+        const double* w = partials + (long long)b * NPART;
+        for (int s = 0; s < 6; ++s) acc[s] = dd_add(acc[s], DD{w[2 * s], w[2 * s + 1]});
+    }
+    DD r[6];
+    for (int s = 0; s < 6; ++s) r[s] = block_reduce_dd(acc[s], sh);
+    if (threadIdx.x == 0) {
+        // partial slots: bcO, ckO, ckCr, clipO, clipCr, clipped
+        // budget slots:  nucO0 nucCr1 ckO2 ckCr3 clipO4 clipCr5 bcO6 bcCr7
+        const int map[5] = {6, 2, 3, 4, 5};
+        for (int s = 0; s < 5; ++s) {
+            DD cur{ctl->budget[2 * map[s]], ctl->budget[2 * map[s] + 1]};
+            cur = dd_add(cur, r[s]);
+            ctl->budget[2 * map[s]] = cur.hi;
+            ctl->budget[2 * map[s] + 1] = cur.lo;
+        }
+        ctl->clipped += (long long)r[5].hi;
+        ctl->cur ^= 1;
+        ctl->steps_done += 1;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_commit(Bufs<double, double> B, int nparts) {
+    DevCtl* ctl = B.ctl;
+    if (ctl->failed) return;
+    commit_body(B.partials, ctl, nparts);
 }
 
 // ---------------------------------------------------------------------------
@@ -822,95 +906,130 @@ __global__ __launch_bounds__(TILE) void k_eta(Par P, Bufs<TE, TM> B) {
     if (bad != NO_KEY) atomicMin(&ctl->eta_key, bad);
 }
 
-// eta trial of all NG fields of one cell (eta_trial, 2D:347-359, SL:514-526)
-// and the next-step mask sums (2D:332), for the X-marching eta kernels: the Y/Z
-// neighbours sit in LDS (one plane tile of PL values per field; o*: offsets
-// of the cell's neighbours in it), the X neighbours and the cell itself in
-// registers. The same operation order as k_eta in binary64.
-template <int PREC, int NG, class TE, class TM>
-__device__ __forceinline__ void eta_cell_update(const Par& P, const Bufs<TE, TM>& B, const TE* lds, int PL, int oym,
-                                                int oyp, int ozm, int ozp, const TE* prev, const TE* cv,
-                                                const TE* nxv, long long c, int cur_i, int nxt_i, bool first_pass,
-                                                bool last_pass, long long key0, __amdgpu_buffer_rsrc_t rout,
-                                                unsigned fs, unsigned long long& bad) {
-    using T = TE;
-    const unsigned cb = (unsigned)c * (unsigned)sizeof(T);
-    const double pp = B.p[cur_i][c];
+// Trial value of field g at one cell for the X-marching eta kernels
+// (eta_trial, 2D:347-359, SL:514-526): the Y/Z neighbours sit in LDS (one plane
+// tile of PL values per field; o*: offsets of the cell's neighbours in it), the
+// X neighbours and the cell itself in registers. binary64: the operation order
+// of k_eta (and of the reference); mixed: binary32 with the cell's force factor.
+// Per-cell inputs of the eta trial besides eta itself: phi and S of the
+// step's input state (binary64) or the cell's force factor (mixed), and the
+// running mask sums S, P of the fields of earlier passes. Loaded ahead of the
+// plane's barrier so that their latency overlaps the LDS staging.
+template <int PREC, class TE>
+struct EtaCell {
+    using V = typename std::conditional<PREC == PREC_FP64, double, float>::type;
+    double pp, S, dtl;          // binary64
+    float coef, fdtl, K, W2;    // binary32
+    V S2, pr;                   // running sums entering this pass
+};
+template <int PREC, class TE, class TM>
+__device__ __forceinline__ EtaCell<PREC, TE> eta_cell_coefs(const Par& P, const Bufs<TE, TM>& B, long long c,
+                                                            int cur_i, int nxt_i, bool first_pass) {
+    using V = typename EtaCell<PREC, TE>::V;
+    EtaCell<PREC, TE> e;
     if constexpr (PREC == PREC_FP64) {
-        const double S = (double)B.s2[cur_i][c];
-        double S2 = first_pass ? 0. : (double)B.s2[nxt_i][c];
-        double pr = first_pass ? 0. : (double)B.pairs[nxt_i][c];
-        const double dtl = P.dt * P.etaL;
+        e.pp = B.p[cur_i][c];
+        e.S = (double)B.s2[cur_i][c];
+        e.dtl = P.dt * P.etaL;
+    } else {
+        e.coef = B.ecoef ? B.ecoef[c] : eta_coef32(P, (float)B.s2[cur_i][c], (float)B.p[cur_i][c]);
+        e.fdtl = (float)(P.dt * P.etaL);
+        e.K = (float)P.etaK;
+        e.W2 = 2.f * (float)P.etaW;
+    }
+    e.S2 = first_pass ? (V)0 : (V)B.s2[nxt_i][c];
+    e.pr = first_pass ? (V)0 : (V)B.pairs[nxt_i][c];
+    return e;
+}
+template <int PREC, class TE>
+__device__ __forceinline__ typename std::conditional<PREC == PREC_FP64, double, float>::type
+eta_value(const Par& P, const EtaCell<PREC, TE>& e, const TE* L, int oym, int oyp, int ozm, int ozp, TE pv, TE cc,
+          TE nv) {
+    if constexpr (PREC == PREC_FP64) {
+        const double ec = (double)cc;
+        const double eyp = (double)L[oyp];
+        const double eym = (double)L[oym];
+        const double ezp = (double)L[ozp];
+        const double ezm = (double)L[ozm];
+        const double xy = ((double)nv - 2. * ec + (double)pv) / (P.dx * P.dx) + (eyp - 2. * ec + eym) / (P.dy * P.dy);
+        const double lp = xy + (ezp - 2. * ec + ezm) / (P.dz * P.dz);
+        const double dF = ec * (P.etaW * (e.S - (1. - h64(e.pp)) + 2. * (e.S - ec * ec)) + P.etaA * h64(e.pp));
+        return ec - e.dtl * (dF - P.etaK * lp);
+    } else {
+#pragma clang fp contract(fast)
+        const float ec = (float)cc;
+        const float lp = ((float)nv - 2.f * ec + (float)pv) * P.fdx2i + ((float)L[oyp] - 2.f * ec + (float)L[oym]) * P.fdy2i
+                       + ((float)L[ozp] - 2.f * ec + (float)L[ozm]) * P.fdz2i;
+        const float dF = ec * (e.coef - e.W2 * ec * ec);
+        return ec - e.fdtl * (dF - e.K * lp);
+    }
+}
+
+// eta trial of all NG fields of one cell and the next-step mask sums (2D:332).
+// The range test (2D:351: reject a non-finite value or one outside
+// [-1e-12, 1 + 1e-12]) is one comparison pair per field; a cell that fails it
+// recomputes its values (same inputs, same code) to find its first failing
+// field for the report, so the common path carries no key arithmetic.
+template <int PREC, int NG, class TE, class TM>
+__device__ __forceinline__ void eta_cell_update(const Par& P, const Bufs<TE, TM>& B, const EtaCell<PREC, TE>& e,
+                                                const TE* lds, int PL, int oym, int oyp, int ozm, int ozp,
+                                                const TE* prev, const TE* cv, const TE* nxv, long long c, int nxt_i,
+                                                bool last_pass, long long key0, __amdgpu_buffer_rsrc_t rout,
+                                                unsigned fs, unsigned long long& bad, bool active) {
+    using T = TE;
+    using V = typename EtaCell<PREC, TE>::V;
+    constexpr V lo = (V)-1.e-12, hi = (V)1. + (V)1.e-12;
+    // Every lane computes; a lane past the mesh edge of a partial tile (whose
+    // values are not the trial) stores through an offset beyond the buffer
+    // resource's range, which the hardware discards, so the common path has no
+    // divergent branch around the field loop.
+    const unsigned cb = active ? (unsigned)c * (unsigned)sizeof(T) : 0xFFFFFFF0u;
+    V S2 = e.S2, pr = e.pr;
+    bool ok = true;
 #pragma unroll
-        for (int g = 0; g < NG; ++g) {
-            const T* L = lds + g * PL;
-            const double ec = (double)cv[g];
-            const double eyp = (double)L[oyp];
-            const double eym = (double)L[oym];
-            const double ezp = (double)L[ozp];
-            const double ezm = (double)L[ozm];
-            const double xy = ((double)nxv[g] - 2. * ec + (double)prev[g]) / (P.dx * P.dx)
-                            + (eyp - 2. * ec + eym) / (P.dy * P.dy);
-            const double lp = xy + (ezp - 2. * ec + ezm) / (P.dz * P.dz);
-            const double dF = ec * (P.etaW * (S - (1. - h64(pp)) + 2. * (S - ec * ec)) + P.etaA * h64(pp));
-            const double v = ec - dtl * (dF - P.etaK * lp);
-            if (!isfinite(v) || v < -1.e-12 || v > 1. + 1.e-12) {
+    for (int g = 0; g < NG; ++g) {
+        const V v = eta_value<PREC>(P, e, lds + g * PL, oym, oyp, ozm, ozp, prev[g], cv[g], nxv[g]);
+        if constexpr (PREC == PREC_FP64) {
+            // binary64 keeps the key per field: the recomputation of the slow
+            // path would hold more registers than the comparison costs
+            if (active && !(v >= lo && v <= hi)) {
                 const unsigned long long key = (unsigned long long)(key0 + (long long)g * P.ncell + c);
                 bad = bad < key ? bad : key;
             }
-            buf_store<T>(rout, cb, (unsigned)g * fs, (TE)v);
+        } else {
+            ok = ok & (v >= lo) & (v <= hi);   // false for NaN as well
+        }
+        buf_store<T>(rout, cb, (unsigned)g * fs, (TE)v);
+        if constexpr (PREC == PREC_FP64) {
             const double q = v * v;
             pr += S2 * q;
             S2 += q;
-            if ((g & 3) == 3) __builtin_amdgcn_sched_barrier(0);
-        }
-        st_nt(&B.s2[nxt_i][c], (TE)S2);
-        st_nt(&B.pairs[nxt_i][c], (TE)pr);
-        if (last_pass) {
-            const double s16 = 16. * pr;
-            st_nt(&B.gb[nxt_i][c], (TE)(P.eta_scale * (s16 < 1. ? s16 : 1.)));
-        }
-    } else {
+        } else {
 #pragma clang fp contract(fast)
-        const float S = (float)B.s2[cur_i][c];
-        const float hphi = h32((float)pp);
-        const float base = (float)P.etaW * (S - (1.f - hphi)) + 2.f * (float)P.etaW * S;
-        const float A = (float)P.etaA * hphi;
-        const float dtl = (float)(P.dt * P.etaL);
-        const float K = (float)P.etaK;
-        const float W2 = 2.f * (float)P.etaW;
-        float S2 = first_pass ? 0.f : (float)B.s2[nxt_i][c];
-        float pr = first_pass ? 0.f : (float)B.pairs[nxt_i][c];
-#pragma unroll
-        for (int g = 0; g < NG; ++g) {
-            const T* L = lds + g * PL;
-            const float ec = (float)cv[g];
-            const float eyp = (float)L[oyp];
-            const float eym = (float)L[oym];
-            const float ezp = (float)L[ozp];
-            const float ezm = (float)L[ozm];
-            const float lp = ((float)nxv[g] - 2.f * ec + (float)prev[g]) * P.fdx2i
-                           + (eyp - 2.f * ec + eym) * P.fdy2i + (ezp - 2.f * ec + ezm) * P.fdz2i;
-            const float dF = ec * (base - W2 * ec * ec + A);
-            const float v = ec - dtl * (dF - K * lp);
-            if (!isfinite(v) || v < -1.e-12f || v > 1.f + 1.e-12f) {
-                const unsigned long long key = (unsigned long long)(key0 + (long long)g * P.ncell + c);
-                bad = bad < key ? bad : key;
-            }
-            buf_store<T>(rout, cb, (unsigned)g * fs, (TE)v);
             const float q = v * v;
             pr += S2 * q;
             S2 += q;
-            // keep the scheduler from hoisting the LDS reads of all fields at
-            // once (register pressure): groups of four fields
-            if ((g & 3) == 3) __builtin_amdgcn_sched_barrier(0);
         }
-        st_nt(&B.s2[nxt_i][c], (TE)S2);
-        st_nt(&B.pairs[nxt_i][c], (TE)pr);
-        if (last_pass) {
-            const float s16 = 16.f * pr;
-            st_nt(&B.gb[nxt_i][c], (TE)((float)P.eta_scale * (s16 < 1.f ? s16 : 1.f)));
+        // keep the scheduler from hoisting the LDS reads of all fields at
+        // once (register pressure): groups of four fields
+        if ((g & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+    if (!active) return;
+    if (PREC != PREC_FP64 && !ok) {
+        for (int g = 0; g < NG; ++g) {
+            const V v = eta_value<PREC>(P, e, lds + g * PL, oym, oyp, ozm, ozp, prev[g], cv[g], nxv[g]);
+            if (!(v >= lo && v <= hi)) {
+                const unsigned long long key = (unsigned long long)(key0 + (long long)g * P.ncell + c);
+                bad = bad < key ? bad : key;
+                break;
+            }
         }
+    }
+    st_nt(&B.s2[nxt_i][c], (TE)S2);
+    if (!last_pass) st_nt(&B.pairs[nxt_i][c], (TE)pr);   // the next pass's running sum
+    if (last_pass) {
+        const V s16 = (V)16 * pr;
+        st_nt(&B.gb[nxt_i][c], (TE)((V)P.eta_scale * (s16 < (V)1 ? s16 : (V)1)));
     }
 }
 
@@ -1028,11 +1147,11 @@ __global__ __launch_bounds__(64 * TY) void k_eta_march(Par P, Bufs<TE, TM> B, in
                 lds[g * PL + ly * W + lz] = hv[u];
             }
         }
+        const EtaCell<PREC, TE> ec = eta_cell_coefs<PREC>(P, B, xo + rc, cur_i, nxt_i, first_pass);
         __syncthreads();
-        if (active)
-            eta_cell_update<PREC, NG>(P, B, lds, PL, ty * W + tz + 1, (ty + 2) * W + tz + 1, (ty + 1) * W + tz,
-                                      (ty + 1) * W + tz + 2, prev, cv, nx_, xo + rc, cur_i, nxt_i, first_pass,
-                                      last_pass, key0, rout, fs, bad);
+        eta_cell_update<PREC, NG>(P, B, ec, lds, PL, ty * W + tz + 1, (ty + 2) * W + tz + 1, (ty + 1) * W + tz,
+                                  (ty + 1) * W + tz + 2, prev, cv, nx_, xo + rc, nxt_i, last_pass, key0, rout, fs,
+                                  bad, active);
         __syncthreads();
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
@@ -1051,8 +1170,14 @@ __global__ __launch_bounds__(64 * TY) void k_eta_march(Par P, Bufs<TE, TM> B, in
 // halo cell is a separate cache-line request: as many requests as the interior,
 // most of them missing L2. Same arithmetic as k_eta_march (eta_cell_update).
 // ---------------------------------------------------------------------------
+// Waves per SIMD the register allocation must allow: the binary32 pass of 16
+// fields is held to 128 VGPRs, so two 512-thread blocks share a CU (measured
+// faster than one block with the unconstrained 139 VGPRs); others unconstrained.
+template <int PREC, int NG>
+constexpr int eta_rows_waves() { return PREC == PREC_MIXED && NG == 16 ? 4 : 1; }
 template <int PREC, int NG, int NZ, class TE, class TM>
-__global__ __launch_bounds__(512) void k_eta_rows(Par P, Bufs<TE, TM> B, int XS, int g0) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(eta_rows_waves<PREC, NG>()))) void k_eta_rows(
+    Par P, Bufs<TE, TM> B, int XS, int g0) {
     using T = TE;
     constexpr int TY = 512 / NZ;
     static_assert((2 * NG) % TY == 0, "halo rows x fields must split evenly over the thread rows");
@@ -1103,21 +1228,40 @@ __global__ __launch_bounds__(512) void k_eta_rows(Par P, Bufs<TE, TM> B, int XS,
             cv[g] = buf_load<T>(rin, cell_bytes((long long)x0 * plane + rc), (unsigned)g * fs);
         }
     }
-    unsigned long long bad = NO_KEY;
-    for (int x = x0; x < x1; ++x) {
-        const long long xo = (long long)x * plane;
-        const long long xn = (long long)(x == P.nx - 1 ? x : x + 1) * plane;
-        // halo rows of plane x, then the next plane's interior
-        // (a wave lies inside one row, as nz is a multiple of 64: ty, and so the
-        // field of each pair, is uniform over the wave)
-        T hv[HC];
+    // Halo rows of plane X (a wave lies inside one row, as nz is a multiple of
+    // 64: ty, and so the field of each (row, field) pair, is uniform over the
+    // wave). In binary32 they are loaded one plane ahead, together with the
+    // next plane's interior: the Y-neighbour blocks load the same rows as their
+    // interior at that moment, so the request finds the line in L2 (or merges
+    // with the neighbour's miss) instead of fetching it from HBM a plane later.
+    // In binary64 the registers that holding them across the plane would take
+    // cost a resident block per CU, which is worth more: loaded in their plane.
+    constexpr bool AHEAD = PREC == PREC_MIXED;
+    T hv[HC];
+    auto load_halo = [&](int X) {
+        const long long xo = (long long)X * plane;
 #pragma unroll
         for (int k = 0; k < HC; ++k) {
             const int pr = __builtin_amdgcn_readfirstlane(ty + k * TY), g = pr >> 1;
             hv[k] = buf_load<T>(rin, cell_bytes(xo + ((pr & 1) ? hrow1 : hrow0)), (unsigned)g * fs);
         }
+    };
+    if (AHEAD) load_halo(x0);
+    unsigned long long bad = NO_KEY;
+    for (int x = x0; x < x1; ++x) {
+        const long long xo = (long long)x * plane;
+        const int xq = x == P.nx - 1 ? x : x + 1;
+        const long long xn = (long long)xq * plane;
+        // binary32: the cell's own inputs of plane x, first of this iteration's loads
+        EtaCell<PREC, TE> ec;
+        if (AHEAD) ec = eta_cell_coefs<PREC>(P, B, xo + rc, cur_i, nxt_i, first_pass);
+        if (!AHEAD) {
+            load_halo(x);
 #pragma unroll
-        for (int g = 0; g < NG; ++g) nx_[g] = buf_load<T>(rin, cell_bytes(xn + rc), (unsigned)g * fs);
+            for (int g = 0; g < NG; ++g) nx_[g] = buf_load<T>(rin, cell_bytes(xn + rc), (unsigned)g * fs);
+        }
+        // plane x (interior and halo) into LDS, then the next plane's interior
+        // and halo rows into the freed registers
 #pragma unroll
         for (int g = 0; g < NG; ++g) lds[g * PL + own] = cv[g];
 #pragma unroll
@@ -1125,10 +1269,15 @@ __global__ __launch_bounds__(512) void k_eta_rows(Par P, Bufs<TE, TM> B, int XS,
             const int pr = ty + k * TY, g = pr >> 1;
             lds[g * PL + ((pr & 1) ? (TY + 1) * nz : 0) + z] = hv[k];
         }
+        if (AHEAD) {
+#pragma unroll
+            for (int g = 0; g < NG; ++g) nx_[g] = buf_load<T>(rin, cell_bytes(xn + rc), (unsigned)g * fs);
+            if (x + 1 < x1) load_halo(xq);
+        }
         __syncthreads();
-        if (active)
-            eta_cell_update<PREC, NG>(P, B, lds, PL, oym, oyp, ozm, ozp, prev, cv, nx_, xo + rc, cur_i, nxt_i,
-                                      first_pass, last_pass, key0, rout, fs, bad);
+        if (!AHEAD) ec = eta_cell_coefs<PREC>(P, B, xo + rc, cur_i, nxt_i, first_pass);
+        eta_cell_update<PREC, NG>(P, B, ec, lds, PL, oym, oyp, ozm, ozp, prev, cv, nx_, xo + rc, nxt_i, last_pass,
+                                  key0, rout, fs, bad, active);
         __syncthreads();
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
@@ -1137,45 +1286,6 @@ __global__ __launch_bounds__(512) void k_eta_rows(Par P, Bufs<TE, TM> B, int XS,
         }
     }
     if (bad != NO_KEY) atomicMin(&ctl->eta_key, bad);
-}
-
-// ---------------------------------------------------------------------------
-// k_commit: all-or-nothing acceptance (collective_error + swap, 2D:594-596;
-// budget restore on rejection, SL:812-820). One block.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_commit(Bufs<double, double> B, int nparts) {
-    DevCtl* ctl = B.ctl;
-    if (ctl->failed) return;
-    if (ctl->pde_key != NO_KEY || ctl->eta_key != NO_KEY) {
-        if (threadIdx.x == 0) {
-            ctl->failed = 1;
-            ctl->fail_kind = ctl->pde_key != NO_KEY ? 1 : 2;
-        }
-        return;
-    }
-    __shared__ DD sh[16];
-    DD acc[6];
-    for (int s = 0; s < 6; ++s) acc[s] = DD{0., 0.};
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
-        const double* w = B.partials + (long long)b * NPART;
-        for (int s = 0; s < 6; ++s) acc[s] = dd_add(acc[s], DD{w[2 * s], w[2 * s + 1]});
-    }
-    DD r[6];
-    for (int s = 0; s < 6; ++s) r[s] = block_reduce_dd(acc[s], sh);
-    if (threadIdx.x == 0) {
-        // partial slots: bcO, ckO, ckCr, clipO, clipCr, clipped
-        // budget slots:  nucO0 nucCr1 ckO2 ckCr3 clipO4 clipCr5 bcO6 bcCr7
-        const int map[5] = {6, 2, 3, 4, 5};
-        for (int s = 0; s < 5; ++s) {
-            DD cur{ctl->budget[2 * map[s]], ctl->budget[2 * map[s] + 1]};
-            cur = dd_add(cur, r[s]);
-            ctl->budget[2 * map[s]] = cur.hi;
-            ctl->budget[2 * map[s] + 1] = cur.lo;
-        }
-        ctl->clipped += (long long)r[5].hi;
-        ctl->cur ^= 1;
-        ctl->steps_done += 1;
-    }
 }
 
 // ---------------------------------------------------------------------------
