@@ -32,30 +32,21 @@ def _pkg():
     return importlib.import_module(PKG_NAME)
 
 
+def _wl():
+    return importlib.import_module(PKG_NAME + ".workloads")
+
+
 def build_state(nicr, shape, n_seeds, n_eta, seed):
-    syn = importlib.import_module(PKG_NAME + ".synthetic")
+    """The 3D evolving-grain workload's state (host arrays)."""
     t0 = time.time()
-    labels, dist, _ = syn.voronoi_labels(shape, n_seeds, seed=seed)
-    colour = syn.colour_seeds(labels, n_seeds, n_eta)
-    o, cr, p = syn.noisy_alloy(shape, 0.20, 0.005, seed=seed)
-    eta = syn.eta_from_labels(labels, dist, colour, n_eta)
+    state = _wl().evolving_grain_state(shape, n_seeds, n_eta, seed)
     print(f"[bench] state {shape} with {n_seeds} seeds -> {n_eta} eta built in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
-    return o, cr, p, eta
+    return state
 
 
 def make_model(nicr, shape, n_eta, precision, state):
-    o, cr, p, eta = state
-    cfg = nicr.make_config(3, nx=shape[0], ny=shape[1], nz=shape[2], grain_evolution=True, n_eta=n_eta,
-                           bc="concentration", bounds="legacy", ck="legacy", nucleation="ksp",
-                           precision=precision)
-    m = nicr.Model(cfg)
-    m.set("o", o)
-    m.set("cr", cr)
-    m.set("p", p)
-    for g in range(n_eta):
-        m.set(g, eta[g])
-    return m
+    return _wl().evolving_grain_model(precision, state, n_eta)
 
 
 def measured_traffic(kernel):
@@ -373,6 +364,9 @@ def main():
     ap.add_argument("--eta", type=int, default=32)
     ap.add_argument("--seeds", type=int, default=64)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--run-steps", type=int, default=5000,
+                    help="steps of the whole-configuration run with the Ksp schedule, mixed and FP64 (0: skip)")
+    ap.add_argument("--no-static", action="store_true", help="skip the 128^3 static-grain error run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 2D ensemble measurement")
     ap.add_argument("--ensemble-only", action="store_true", help="only the 2D ensemble measurement")
@@ -449,20 +443,49 @@ def main():
             fp64["dram"] = {"bytes_per_step": t64["hbm_bytes_per_step"],
                             "GBs": round(t64["hbm_bytes_per_step"] / ms64 / 1e6, 1),
                             "source": os.path.relpath(path64, REPO)}
+    # The whole configuration: args.run_steps steps of the reference's run loop
+    # with its Ksp schedule (scans before steps 500, 1000, ...), mixed and
+    # binary64 from the same state, timed end to end (scans and budget
+    # reductions included), and the mixed run's error against the binary64 run.
+    config_run = mixed_error = None
+    if args.run_steps > 0:
+        def log(msg):
+            print(f"[bench] config run: {msg}", file=sys.stderr, flush=True)
+        res = _wl().mixed_vs_fp64(lambda prec: make_model(nicr, shape, args.eta, prec, state), args.run_steps, log)
+        step_b = cells * (48 + 8 * args.eta)
+        config_run = {"steps": args.run_steps, "scans": res["mixed"]["scans"],
+                      "inserted_cells": res["mixed"]["inserted_cells"]}
+        for prec in ("mixed", "fp64"):
+            sec = res[prec]["seconds"]
+            config_run[prec] = {"seconds": sec, "value": round(cells * args.run_steps / sec / 1e9, 4),
+                                "ms_per_step": round(sec / args.run_steps * 1e3, 4)}
+        config_run["mixed"]["hbm_fraction"] = round(step_b * args.run_steps / res["mixed"]["seconds"] / 1e9
+                                                    / HBM_PEAK_GBS, 4)
+        mixed_error = {f"evolving_{args.n}cube_{args.eta}eta_{args.run_steps}steps":
+                       {k: res[k] for k in ("error", "totals_rel_vs_fp64")} |
+                       {"closure_mixed": res["mixed"]["closure"], "closure_fp64": res["fp64"]["closure"]}}
+        if not args.no_static:
+            rs = _wl().mixed_vs_fp64(lambda prec: _wl().static_grain_model(prec), 2000, log)
+            mixed_error["static_128cube_16grains_2000steps"] = (
+                {k: rs[k] for k in ("error", "totals_rel_vs_fp64")} |
+                {"closure_mixed": rs["mixed"]["closure"], "closure_fp64": rs["fp64"]["closure"],
+                 "seconds": {"mixed": rs["mixed"]["seconds"], "fp64": rs["fp64"]["seconds"]},
+                 "value": {p: round(128 ** 3 * 2000 / rs[p]["seconds"] / 1e9, 4) for p in ("mixed", "fp64")}})
     del state
 
-    # Roofline of the dominant kernel, k_eta_march (one launch per pass of eta
-    # fields). Algorithmic bytes per launch: 8 B per eta field per cell (FP32
-    # read once + written once) x the fields of one pass x the cells; duration:
-    # mean per launch from HIP events on the context stream.
-    eta_ms, eta_count = kt.get("k_eta_march", (float("nan"), 0))
+    # Roofline of the dominant kernel, k_eta_rows (one launch per pass of 16
+    # eta fields). Algorithmic bytes per launch: 8 B per eta field per cell
+    # (binary32 read once + written once) x the fields of one pass x the cells;
+    # duration: mean per launch from HIP events on the context stream.
+    eta_kernel = "k_eta_rows" if "k_eta_rows" in kt else "k_eta_march"
+    eta_ms, eta_count = kt.get(eta_kernel, (float("nan"), 0))
     launches_per_step = eta_count / args.steps if eta_count else float("nan")
     eta_bytes = cells * 8 * args.eta / launches_per_step
     achieved = eta_bytes / (eta_ms * 1e-3) / 1e9
     pde_ms = kt.get("k_pde_march", (float("nan"), 0))[0]
-    pde_bytes = cells * 56   # o, cr, phi FP64 read + write, gb and S FP32 read
+    pde_bytes = cells * 52   # o, cr, phi binary64 read + write, gb binary32 read
     step_bytes = cells * (48 + 8 * args.eta)   # FP64 o/cr/phi + FP32 eta, read + write
-    traffic = measured_traffic("k_eta_march")
+    traffic = measured_traffic(eta_kernel)
     out = {
         "metric": "Gcell-steps/s @256^3 mixed vs FP64 GPU, % HBM peak; surrogate speedup @1e-3 L2",
         "value": round(value, 4),
@@ -482,7 +505,7 @@ def main():
         "step_hbm_fraction": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v[0], 5) for k, v in kt.items()},
         "kernel_launches_per_step": {k: round(v[1] / args.steps, 3) for k, v in kt.items()},
-        "roofline": {"bound": "hbm", "kernel": "k_eta_march", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": eta_kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes_per_launch": int(eta_bytes), "ms_per_launch": round(eta_ms, 5),
                      "traffic": traffic},
@@ -491,6 +514,8 @@ def main():
                          "unit": "GB/s", "frac": round(pde_bytes / (pde_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                          "algorithmic_bytes_per_launch": int(pde_bytes)},
         "fp64": fp64,
+        "config_run": config_run,
+        "mixed_error": mixed_error,
         "copy_probe_GBs": round(copy_gbs, 1),
     }
     if not args.no_ensemble:
