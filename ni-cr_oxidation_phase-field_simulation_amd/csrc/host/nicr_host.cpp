@@ -170,7 +170,10 @@ HostConfig parse_args(int dim, int argc, char** argv, std::ostream& out) {
         c.max_eta_fields < 1)
         throw std::runtime_error("Invalid eta parameter bounds.");
     if (dim == 3) {
-        if (h.output_format != "vti" && h.output_format != "vtk") throw std::runtime_error("--output-format expects vti or vtk.");
+        // "vtk-blocks" (no counterpart in mpi3d-slab): the legacy VTK array set of
+        // the mpi3d-blocks program (BK:780-817)
+        if (h.output_format != "vti" && h.output_format != "vtk" && h.output_format != "vtk-blocks")
+            throw std::runtime_error("--output-format expects vti or vtk.");
         if (h.output_precision != "float32" && h.output_precision != "float64")
             throw std::runtime_error("--output-precision expects float32 or float64.");
         if (h.compression_level < 0 || h.compression_level > 9) throw std::runtime_error("--compression-level must lie in [0,9].");
@@ -221,7 +224,8 @@ HostModel::HostModel(const HostConfig& cfg) : hc(cfg) {
     gb.assign(ncell, 0.);
     grain.assign(ncell, 0.);
     voronoi();
-    if (dim == 2) gb_initial = gb;
+    // the Voronoi mask before eta takes over (2D's vtk and the vtk-blocks layout write it)
+    if (dim == 2 || (c.grains && c.grain_evolution)) gb_initial = gb;
     const bool layer = hc.bc == "concentration";
     for (int i = 1; i <= nx; ++i)
         for (int j = 1; j <= ny; ++j)
@@ -979,6 +983,35 @@ void HostModel::write_vtk(const std::string& path) const {
                 out << "\n";
             });
         };
+        if (hc.output_format == "vtk-blocks") {
+            // the array set and order of mpi3d-blocks (BK:780-817), that of the 2D program
+            write("phi", [&](std::size_t q) { return p[q]; });
+            write("conc_O", [&](std::size_t q) { return o[q]; });
+            write("conc_Cr", [&](std::size_t q) { return cr[q]; });
+            write("conc_Ni", [&](std::size_t q) { return 1. - o[q] - cr[q]; });
+            write("grain_id", [&](std::size_t q) { return has_eta() ? double(current_grain(q)) : grain[q]; });
+            write("GB_mask", [&](std::size_t q) { return gb[q]; });
+            write("GB_active", [&](std::size_t q) { return (1. - h(p[q])) * gb[q]; });
+            write("mu_O", [&](std::size_t q) { return muO[q]; });
+            write("mu_Cr", [&](std::size_t q) { return muCr[q]; });
+            write("M_O", [&](std::size_t q) { return mo[q]; });
+            write("M_Cr", [&](std::size_t q) { return mc[q]; });
+            if (has_eta()) {
+                write("grain_id_initial", [&](std::size_t q) { return grain[q]; });
+                write("GB_initial", [&](std::size_t q) { return gb_initial[q]; });
+                write("GB_eta_indicator", [&](std::size_t q) { return std::min(1., 16. * eta_pairs[q]); });
+                write("metal_fraction", [&](std::size_t q) { return 1. - h(p[q]); });
+                write("eta_sum_sq", [&](std::size_t q) { return eta_s2[q]; });
+                write("active_grain_id", [&](std::size_t q) { return double(current_grain(q)); });
+                if (hc.eta_output == "all")
+                    for (std::size_t g = 0; g < eta.size(); ++g) {
+                        const std::string name = "eta_" + std::to_string(g + 1);
+                        write(name.c_str(), [&](std::size_t q) { return eta[g][q]; });
+                    }
+            }
+            if (!f) throw std::runtime_error("Output write failed: " + path);
+            return;
+        }
         write("phi", [&](std::size_t q) { return p[q]; });
         write("conc_O", [&](std::size_t q) { return o[q]; });
         write("conc_Cr", [&](std::size_t q) { return cr[q]; });
@@ -1306,13 +1339,14 @@ int nicr_host_write(void* h, int what, const char* path, int step, const char* o
             need(m.muO, "mu_O");
             need(m.muCr, "mu_Cr");
         }
-        if (what == 3 && m.dim == 2) {
+        const bool full_vtk = what == 3 && (m.dim == 2 || m.hc.output_format == "vtk-blocks");
+        if (full_vtk) {
             need(m.mo, "M_O");
             need(m.mc, "M_Cr");
         }
-        if ((what == 2 || (what == 3 && m.dim == 2)) && m.has_eta()) {
+        if ((what == 2 || full_vtk) && m.has_eta()) {
             need(m.eta_s2, "eta_s2");
-            if (m.dim == 2) need(m.eta_pairs, "eta_pairs");
+            if (full_vtk) need(m.eta_pairs, "eta_pairs");
         }
         if (what == 0) {
             m.hc.out = path;

@@ -156,3 +156,23 @@ def test_nicr3d_grain_rows_from_device_terms(tmp_path, precision):
         assert sha(outs["device"] / f) == sha(outs["host"] / f), f
     rows = open(outs["device"] / "grain_diagnostics.csv").read().splitlines()
     assert len(rows) == 4
+
+
+BLOCKS = json.load(open(os.path.join(REPO, "tests", "golden", "blocks_regression.json")))
+
+
+@pytest.mark.parametrize("case", sorted(BLOCKS["cases"]))
+def test_nicr3d_reproduces_blocks_regression(tmp_path, case):
+    """The 10 mpi3d-blocks regression cases (mpi3d-blocks/tests/verify_mpi.py:119-133,
+    hashes from mpi3d-blocks/tests/logs/output_regression_summary.json) through
+    nicr3d on the GPU with the blocks program's VTK array set (--output-format
+    vtk-blocks): fields_0.vtk, fields_30.vtk, grains.csv and, with evolving
+    grains, both grain CSVs byte for byte."""
+    spec = BLOCKS["cases"][case]
+    out = tmp_path / "run"
+    args = BLOCKS["common_args"] + spec["args"] + ["--output-format", "vtk-blocks", "--out", str(out)]
+    print(f"[blocks] nicr3d {case}", flush=True)
+    r = subprocess.run([os.path.join(PKG, "nicr3d"), *args], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr + r.stdout
+    for f, h in spec["sha256"].items():
+        assert sha(out / f) == h, f
