@@ -153,33 +153,13 @@ def cpu_baseline(nicr, n_eta, seed):
                       f"{done} FP64 steps, {t1 - t0:.1f} s on one host thread"}
 
 
-def ensemble_state(shape, cr0, layer_o, rng):
-    """C1-like 2D state: O = 0 with the inlet row at layer_o (Cr 0.182), Cr =
-    cr0 + U(-0.005, 0.005), centre oxide disk of radius 5 cells."""
-    nx, ny = shape
-    o = np.zeros(shape)
-    cr = cr0 + rng.uniform(-0.005, 0.005, size=shape)
-    p = np.zeros(shape)
-    o[0, :] = layer_o
-    cr[0, :] = 0.182
-    i, j = np.meshgrid(np.arange(1, nx + 1), np.arange(1, ny + 1), indexing="ij")
-    disk = (i - (nx + 1) / 2.0) ** 2 + (j - (ny + 1) / 2.0) ** 2 <= 25.0
-    o[disk], cr[disk], p[disk] = 0.56300203, 0.35677915, 1.0
-    return o, cr, p
-
-
 def ensemble_bench(nicr, n_inst, steps, seed):
     """Ensemble of n_inst 2D instances, sides from {64, 128, 256, 512}, Cr in
     U(0.15, 0.30), inlet O supply U(0.5, 1.5) x 0.0023: autotuned graph replay vs
     the same instances stepped one after another as separate contexts."""
-    rng = np.random.default_rng(seed)
-    sides = rng.choice([64, 128, 256, 512], size=n_inst)
-    crs = rng.uniform(0.15, 0.30, size=n_inst)
-    supply = rng.uniform(0.5, 1.5, size=n_inst) * 0.0023
-    insts = [(int(s), int(s), float(lo)) for s, lo in zip(sides, supply)]
-    states = [ensemble_state((s, s), c, lo, rng) for (s, _, lo), c in zip(insts, crs)]
-    cfg = nicr.make_config(2, nx=8, ny=8, grains=False, bc="concentration", bounds="legacy", ck="legacy",
-                           nucleation="off")
+    insts, states = _wl().ensemble_instances(n_inst, seed)
+    sides = [nx for nx, _, _ in insts]
+    cfg = _wl().ensemble_config()
     ens = nicr.Ensemble(cfg, insts)
     for i, st in enumerate(states):
         for name, a in zip(("o", "cr", "p"), st):

@@ -333,6 +333,194 @@ void self_test(int dim) {
     std::cout << "PASS: mu-BC inlet inventory balance.\n";
 }
 
+// ---------------------------------------------------------------------------
+// --self-test, evolving grains: the eta checks of the reference self-tests
+// (2D:1011-1093, SL:1201-1230) on the device step
+// ---------------------------------------------------------------------------
+HostConfig eta_config(int dim) {
+    HostConfig hc;
+    hc.dim = dim;
+    nicr_config_default(&hc.c, dim);
+    nicr_config& c = hc.c;
+    c.grain_evolution = 1;
+    c.grains = 1;
+    c.nx = dim == 2 ? 35 : 23;
+    c.ny = dim == 2 ? 29 : 13;
+    c.nz = dim == 2 ? 1 : 11;
+    c.grain_diameter_grid = dim == 2 ? 16. : 12.;
+    c.bc = NICR_BC_CLOSED;
+    c.ck = NICR_CK_OFF;
+    c.nucleation = NICR_NUC_OFF;
+    c.bounds = NICR_BOUNDS_LEGACY;
+    c.dt = 1.e-10;
+    hc.bc = "closed";
+    hc.ck = "off";
+    hc.nucleation = "off";
+    hc.bounds = "legacy";
+    nicr_config_derive(&c);
+    return hc;
+}
+
+// the device's mask of the host arrays' eta, then the host's discrete grain
+// energy (2D:367-381, SL:546-565) of that state
+long double device_energy(HostModel& m, void* h) {
+    for (std::size_t g = 0; g < m.eta.size(); ++g)
+        check(nicr_upload_field(h, NICR_FIELD_ETA0 + int(g), m.eta[g].data()), "upload eta");
+    check(nicr_upload_field(h, NICR_FIELD_PHI, m.p.data()), "upload phi");
+    m.eta_s2.resize(m.ncell);
+    m.eta_pairs.resize(m.ncell);
+    check(nicr_download_field(h, NICR_FIELD_ETA_S2, m.eta_s2.data()), "download S");
+    check(nicr_download_field(h, NICR_FIELD_ETA_PAIRS, m.eta_pairs.data()), "download P");
+    m.grain_terms_valid = false;
+    return m.grain_energy();
+}
+
+void eta_self_test(int dim) {
+    // (1) the host-built eta state partitions unity after the trip through the device (2D:1015-1018, SL:1202-1205)
+    {
+        HostConfig hc = eta_config(dim);
+        HostModel m(hc);
+        nicr_config dc = hc.c;
+        dc.n_eta = int(m.eta.size());
+        require(dc.n_eta > 1, "several eta fields");
+        Ctx d(dc);
+        upload(m, d.h);
+        std::vector<std::vector<double>> e(m.eta.size(), std::vector<double>(m.ncell));
+        for (std::size_t g = 0; g < e.size(); ++g)
+            check(nicr_download_field(d.h, NICR_FIELD_ETA0 + int(g), e[g].data()), "download eta");
+        for (std::size_t q = 0; q < m.ncell; ++q) {
+            double sum = 0.;
+            for (const auto& f : e) sum += f[q];
+            require(std::abs(sum - 1.) < 1.e-14, "eta initialization partitions unity");
+        }
+        // (2) eta-L = 0 freezes eta (SL:1206-1210; 2D runs the trial, which leaves it unchanged)
+        nicr_config fc = dc;
+        fc.eta_L = 0.;
+        Ctx f(fc);
+        upload(m, f.h);
+        nicr_step_report r{};
+        check(nicr_step(f.h, 5, &r), "step");
+        require(r.status == 0 && r.steps_done == 5, "frozen-eta steps accepted");
+        std::vector<double> v(m.ncell);
+        for (std::size_t g = 0; g < m.eta.size(); ++g) {
+            check(nicr_download_field(f.h, NICR_FIELD_ETA0 + int(g), v.data()), "download eta");
+            require(v == m.eta[g], "eta-L=0 freezes eta");
+        }
+        int n = 0;
+        check(dim == 2 ? nicr_stamp(f.h, 12, 1, 1, &n) : nicr_stamp(f.h, 12, 1, 1, &n), "stamp");
+        require(n > 0, "hard seed inserted");
+        for (std::size_t g = 0; g < m.eta.size(); ++g) {
+            check(nicr_download_field(f.h, NICR_FIELD_ETA0 + int(g), v.data()), "download eta");
+            require(v[m.at(12, 1, 1)] == 0., "hard seeds consume eta even with eta-L=0");
+        }
+    }
+    std::cout << "PASS: eta initialization partitions unity; eta-L=0 freezes eta; seeds consume eta.\n";
+    // (3) the device's eta force is the variational derivative of the discrete
+    // grain energy, boundary faces included (2D:1026-1047, SL:1211-1227): the
+    // force is read off one device step, F = (eta - eta') / (dt L_eta)
+    {
+        HostConfig hc = eta_config(dim);
+        nicr_config& c = hc.c;
+        c.dx = c.dy = c.dz = 1.;
+        if (dim == 2) {
+            c.eta_W = 1.;
+            c.eta_kappa = 2.;
+            c.eta_oxide_penalty = .7;
+        } else {
+            c.nx = 9;
+            c.ny = 7;
+            c.nz = 5;
+            c.grain_diameter_grid = 6.;
+            c.eta_W = 2.;
+            c.eta_kappa = .3;
+        }
+        c.eta_L = .1;
+        c.dt = 1.e-3;
+        nicr_config_derive(&c);
+        HostModel m(hc);
+        for (int i = 1; i <= m.nx; ++i)
+            for (int j = 1; j <= m.ny; ++j)
+                for (int k = 1; k <= m.nz; ++k) {
+                    const std::size_t q = m.at(i, j, k);
+                    const double kk = dim == 3 ? k : 0.;
+                    m.p[q] = .25 + .08 * std::sin(.31 * i + .27 * j + .2 * kk);
+                    for (std::size_t g = 0; g < m.eta.size(); ++g)
+                        m.eta[g][q] = .1 + .04 * std::cos(.2 * i + .3 * j + .7 * g + .5 * kk);
+                }
+        nicr_config dc = c;
+        dc.n_eta = int(m.eta.size());
+        Ctx d(dc);
+        upload(m, d.h);
+        nicr_step_report r{};
+        check(nicr_step(d.h, 1, &r), "step");
+        require(r.status == 0, "force step accepted");
+        std::vector<double> e1(m.ncell);
+        check(nicr_download_field(d.h, NICR_FIELD_ETA0, e1.data()), "download eta");
+        Ctx en(dc);
+        upload(m, en.h);
+        const double eps = 1.e-5, vol = dim == 2 ? c.dx * c.dy : c.dx * c.dy * c.dz;
+        double max_error = 0.;
+        for (int i : {1, (m.nx + 1) / 2, m.nx})
+            for (int j : {1, (m.ny + 1) / 2, m.ny})
+                for (int k : {1, m.nz}) {
+                    const std::size_t q = m.at(i, j, dim == 3 ? k : 1);
+                    const double force = (m.eta[0][q] - e1[q]) / (c.dt * c.eta_L);
+                    const double e0 = m.eta[0][q];
+                    m.eta[0][q] = e0 + eps;
+                    const long double plus = device_energy(m, en.h);
+                    m.eta[0][q] = e0 - eps;
+                    const long double minus = device_energy(m, en.h);
+                    m.eta[0][q] = e0;
+                    const double approx = double((plus - minus) / (2. * eps * vol));
+                    const double err = std::abs(force - approx) / (1. + std::abs(force));
+                    max_error = std::max(max_error, err);
+                    require(err < (dim == 2 ? 1.e-7 : 2.e-7), "grain energy derivative equals the device eta force");
+                    if (dim == 2) break;
+                }
+        std::cout << "ETA CHECK: largest scaled energy-derivative error of the device eta force=" << max_error << "\n";
+    }
+    // (4) closed grain-only energy is nonincreasing under the device step (2D:1064-1080, SL:1228-1230)
+    {
+        HostConfig hc = eta_config(dim);
+        nicr_config& c = hc.c;
+        c.dx = c.dy = c.dz = 1.;
+        c.eta_W = 1.;
+        c.eta_kappa = 2.;
+        c.eta_L = 1.;
+        c.dt = .005;
+        if (dim == 3) c.dt = .002;
+        nicr_config_derive(&c);
+        HostModel m(hc);
+        for (std::size_t q = 0; q < m.ncell; ++q) {
+            m.o[q] = 0.;
+            m.cr[q] = Phys::bulkCr;
+            m.p[q] = 0.;
+        }
+        nicr_config dc = c;
+        dc.n_eta = int(m.eta.size());
+        Ctx d(dc);
+        upload(m, d.h);
+        Ctx en(dc);
+        upload(m, en.h);
+        long double previous = device_energy(m, en.h);
+        const long double first = previous;
+        for (int n = 0; n < 60; ++n) {
+            nicr_step_report r{};
+            check(nicr_step(d.h, 1, &r), "step");
+            require(r.status == 0, "grain-only step accepted");
+            for (std::size_t g = 0; g < m.eta.size(); ++g)
+                check(nicr_download_field(d.h, NICR_FIELD_ETA0 + int(g), m.eta[g].data()), "download eta");
+            const long double now = device_energy(m, en.h);
+            require(now <= previous + 1.e-12L * (1.L + std::abs(previous)), "closed grain-only energy is nonincreasing");
+            previous = now;
+        }
+        require(previous < first, "grain energy decreases");
+        std::cout << "ETA CHECK: closed grain-only energy " << double(first) << " -> " << double(previous)
+                  << " over 60 device steps\n";
+    }
+    std::cout << "PASS: device eta force is the energy variation; grain energy descends.\n";
+}
+
 int run(int argc, char** argv) {
     const int dim = NICR_CLI_DIM;
     PhaseTimes pt;
@@ -341,6 +529,7 @@ int run(int argc, char** argv) {
         if (hc.self_test) {
             std::cout << "GPU SELF-TEST: device step of this build (" << backend_name() << ").\n";
             self_test(dim);
+            eta_self_test(dim);
             return 0;
         }
         if (fs::exists(hc.out) && !fs::is_empty(hc.out)) throw std::runtime_error("Output directory is not empty; choose a new --out.");

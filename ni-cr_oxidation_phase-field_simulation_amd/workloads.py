@@ -143,3 +143,39 @@ def mixed_vs_fp64(make, steps: int, log=None) -> dict:
     for m in runs.values():
         m.close()
     return out
+
+
+def ensemble_state(shape, cr0, layer_o, rng):
+    """One 2D instance of the ensemble workload: O = 0 with the inlet row at
+    layer_o (Cr 0.182), Cr = cr0 + U(-0.005, 0.005), the centre oxide disk of
+    radius 5 cells (the reference's initialize_center_oxide geometry)."""
+    nx, ny = shape
+    o = np.zeros(shape)
+    cr = cr0 + rng.uniform(-0.005, 0.005, size=shape)
+    p = np.zeros(shape)
+    o[0, :] = layer_o
+    cr[0, :] = 0.182
+    i, j = np.meshgrid(np.arange(1, nx + 1), np.arange(1, ny + 1), indexing="ij")
+    disk = (i - (nx + 1) / 2.0) ** 2 + (j - (ny + 1) / 2.0) ** 2 <= 25.0
+    o[disk], cr[disk], p[disk] = syn.OXIDE_O, syn.OXIDE_CR, 1.0
+    return o, cr, p
+
+
+def ensemble_instances(n_inst: int = 64, seed: int = SEED):
+    """The 2D ensemble workload: n_inst instances with sides drawn from
+    {64, 128, 256, 512}, Cr in U(0.15, 0.30), inlet O U(0.5, 1.5) x 0.0023.
+    Returns ([(nx, ny, layer_o)], [(o, cr, p)])."""
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([64, 128, 256, 512], size=n_inst)
+    crs = rng.uniform(0.15, 0.30, size=n_inst)
+    supply = rng.uniform(0.5, 1.5, size=n_inst) * syn.LAYER_O
+    insts = [(int(s), int(s), float(lo)) for s, lo in zip(sides, supply)]
+    states = [ensemble_state((s, s), c, lo, rng) for (s, _, lo), c in zip(insts, crs)]
+    return insts, states
+
+
+def ensemble_config(**kw):
+    """The ensemble's shared 2D configuration (binary64, no grains)."""
+    base = dict(grains=False, bc="concentration", bounds="legacy", ck="legacy", nucleation="off")
+    base.update(kw)
+    return make_config(2, nx=8, ny=8, **base)

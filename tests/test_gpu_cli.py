@@ -176,3 +176,17 @@ def test_nicr3d_reproduces_blocks_regression(tmp_path, case):
     assert r.returncode == 0, r.stderr + r.stdout
     for f, h in spec["sha256"].items():
         assert sha(out / f) == h, f
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_device_self_test(dim):
+    """--self-test of the drop-in programs: the reference's self-test checks
+    re-expressed on the device step (conservation, O switch, seed geometry,
+    rejection, inlet balance, and the eta checks: partition of unity, frozen
+    eta, seeds consuming eta, the device eta force as the variation of the
+    discrete grain energy, energy descent; 2D:1011-1093, SL:1201-1230)."""
+    r = subprocess.run([os.path.join(PKG, f"nicr{dim}d"), "--self-test"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("PASS:") >= 7
+    assert "grain energy descends" in r.stdout
