@@ -157,6 +157,16 @@ int nicr_step(void* ctx, int nsteps, nicr_step_report* report);
 /* ---- oxide insertion ---- */
 /* Model::nucleate (mpi2d:543-550, mpi3d-slab:752-759): ordered Ksp scan. */
 int nicr_nucleate(void* ctx, int* inserted_cells);
+/* nicr_nucleate without device allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The scan's
+ * scratch (per-tile counts, per-plane oxide counts, the candidate list, the
+ * accepted centres) lives in one static 32 MiB device buffer per GPU, so the
+ * mesh must satisfy tiles + nx + ncell + 4160 <= 8 Mi ints (about 8.3 M cells);
+ * larger meshes are refused with NICR_E_INVALID (use nicr_nucleate). Calls that
+ * overlap on one GPU share that buffer and race; results are those of
+ * nicr_nucleate bit for bit. */
+int nicr_nucleate_noalloc(void* ctx, int* inserted_cells);
 /* Model::stamp (mpi2d:503-516, mpi3d-slab:704-724); coordinates are 1-based. */
 int nicr_stamp(void* ctx, int x, int y, int z, int* changed);
 /* Model::initialize_center_oxide (mpi2d:517-542, mpi3d-slab:725-751): one hard

@@ -83,6 +83,7 @@ struct Ctx {
                                    // 0: 8 in binary64, 4 in mixed precision, the faster of each at 256^3)
     int pde_nseg = 0;              // X segments of the marching PDE kernel (NICR_PDE_NSEG; 0: auto)
     int cur = 0;                 // host mirror of ctl->cur
+    int device = 0;              // the HIP device current at nicr_create
     bool mask_dirty = true;
     bool pairs_stale = false;      // the last eta pass of a step leaves P unwritten
     bool refresh_valid = false;
@@ -407,19 +408,37 @@ void launch_stamp(Ctx* c, NucPar N) {
     hipLaunchKernelGGL((k_stamp<DIM, TE>), dim3(1), dim3(1024), 0, c->stream, c->P, N, d, d + 2, d + 4, eta,
                        c->ctl, c->nuc_misc + 1);
 }
+// Scratch of one Ksp scan: per-tile candidate counts (then offsets), per-plane
+// oxide counts, [0] candidate total and [1] inserted cells, accepted centres,
+// the ordered candidate list.
+struct NucScratch {
+    int* tiles;
+    int* planes;
+    int* misc;
+    int* centres;
+    int* list;
+};
+
+__global__ void k_fill_int(int* a, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
 template <int DIM, class TE>
-void launch_nucleate(Ctx* c, NucPar N) {
+void launch_nucleate(Ctx* c, NucPar N, const NucScratch& S, bool fill_kernel) {
     double* const* d = (double* const*)c->dev_obuf;
     TE* eta = c->has_eta() ? (TE*)c->eta[c->eta_evolves() ? c->cur : 0] : nullptr;
-    hipMemsetAsync(c->nuc_planes, 0, (size_t)c->P.nx * sizeof(int), c->stream);
+    if (fill_kernel)
+        hipLaunchKernelGGL(k_fill_int, dim3((c->P.nx + 255) / 256), dim3(256), 0, c->stream, S.planes, c->P.nx, 0);
+    else
+        hipMemsetAsync(S.planes, 0, (size_t)c->P.nx * sizeof(int), c->stream);
     hipLaunchKernelGGL((k_ksp_mark<DIM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P, N.r, d, d + 2, d + 4,
-                       (const DevCtl*)c->ctl, c->nuc_tiles, c->nuc_planes);
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c->stream, c->nuc_tiles, c->P.ntiles, c->nuc_misc);
+                       (const DevCtl*)c->ctl, S.tiles, S.planes);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c->stream, S.tiles, c->P.ntiles, S.misc);
     hipLaunchKernelGGL((k_ksp_compact<DIM>), dim3(c->P.ntiles), dim3(TILE), 0, c->stream, c->P, N.r, d, d + 2,
-                       d + 4, (const DevCtl*)c->ctl, (const int*)c->nuc_tiles, c->nuc_list);
+                       d + 4, (const DevCtl*)c->ctl, (const int*)S.tiles, S.list);
     hipLaunchKernelGGL((k_ksp_resolve<DIM, TE>), dim3(1), dim3(1024), 0, c->stream, c->P, N, d, d + 2, d + 4, eta,
-                       c->ctl, (const int*)c->nuc_list, (const int*)c->nuc_misc, (const int*)c->nuc_planes,
-                       c->nuc_centres, c->nuc_misc + 1);
+                       c->ctl, (const int*)S.list, (const int*)S.misc, (const int*)S.planes, S.centres, S.misc + 1);
 }
 
 void do_mask(Ctx* c, int pairs_only = 0) {
@@ -568,6 +587,10 @@ int nicr_create(const nicr_config* cfg, void** out) {
 
     Ctx* c = new Ctx;
     c->cfg = *cfg;
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        delete c;
+        return fail(NICR_E_DEVICE, "nicr_create: hipGetDevice failed");
+    }
     if (const char* kv = std::getenv("NICR_KERNEL_VARIANT")) c->march_disabled = std::strcmp(kv, "basic") == 0;
     if (const char* ty = std::getenv("NICR_ETA_TY")) c->eta_ty = std::atoi(ty);
     if (const char* ns = std::getenv("NICR_ETA_NSEG")) c->eta_nseg = std::atoi(ns);
@@ -1001,16 +1024,85 @@ int nicr_nucleate(void* h, int* inserted) {
     int rc = ensure_nuc_scratch(c);
     if (rc) return rc;
     NucPar N{0, 0, 0, c->cfg.radius_grid, c->cfg.exclusion_gap_grid, 4096};
+    const NucScratch S{c->nuc_tiles, c->nuc_planes, c->nuc_misc, c->nuc_centres, c->nuc_list};
     if (c->dim == 2) {
-        if (c->te == 8) launch_nucleate<2, double>(c, N);
-        else launch_nucleate<2, float>(c, N);
+        if (c->te == 8) launch_nucleate<2, double>(c, N, S, false);
+        else launch_nucleate<2, float>(c, N, S, false);
     } else {
-        if (c->te == 8) launch_nucleate<3, double>(c, N);
-        else launch_nucleate<3, float>(c, N);
+        if (c->te == 8) launch_nucleate<3, double>(c, N, S, false);
+        else launch_nucleate<3, float>(c, N, S, false);
     }
     HIPCHK(hipGetLastError());
     int n = 0;
     HIPCHK(hipMemcpyAsync(&n, c->nuc_misc + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (n && c->has_eta()) c->mask_dirty = true;
+    c->refresh_valid = false;
+    if (inserted) *inserted = n;
+    return NICR_OK;
+}
+
+// ---- Ksp scan without device allocation (nicr_nucleate_noalloc) ----
+// The scan's scratch lives in one file-scope device array per GPU instead of
+// buffers the context allocates on its first scan. Its size caps the mesh:
+// tiles + planes + 4 + centres + cells ints (each part 256-byte aligned).
+constexpr size_t NOALLOC_INTS = size_t(8) << 20;   // 32 MiB per GPU, held until the process ends
+static __device__ __align__(256) int scratch[NOALLOC_INTS];
+static void* g_scratch_ptr[64] = {};   // device address of `scratch` on each GPU (hipGetSymbolAddress)
+
+static size_t align64(size_t n) { return (n + 63) & ~size_t(63); }
+
+static int noalloc_scratch(Ctx* c, NucScratch* S) {
+    const size_t need = align64((size_t)c->P.ntiles) + align64((size_t)c->P.nx) + 64 + align64(4096) +
+                        align64((size_t)c->ncell);
+    if (need > NOALLOC_INTS)
+        return fail(NICR_E_INVALID, "nicr_nucleate_noalloc: a mesh of " + std::to_string(c->ncell) + " cells needs " +
+                                        std::to_string(need * 4) + " bytes of scan scratch; the static buffer holds " +
+                                        std::to_string(NOALLOC_INTS * 4) + " (use nicr_nucleate)");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(NICR_E_DEVICE, "nicr_nucleate_noalloc: device index out of range");
+    if (dev != c->device)
+        return fail(NICR_E_STATE, "nicr_nucleate_noalloc: the current device is not the context's");
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = ptr;
+    }
+    int* base = static_cast<int*>(g_scratch_ptr[dev]);
+    size_t off = 0;
+    auto take = [&](size_t n) {
+        int* q = base + off;
+        off += align64(n);
+        return q;
+    };
+    S->tiles = take((size_t)c->P.ntiles);
+    S->planes = take((size_t)c->P.nx);
+    S->misc = take(64);
+    S->centres = take(4096);
+    S->list = take((size_t)c->ncell);
+    return NICR_OK;
+}
+
+int nicr_nucleate_noalloc(void* h, int* inserted) {
+    Ctx* c = static_cast<Ctx*>(h);
+    if (!c) return fail(NICR_E_INVALID, "null context");
+    if (inserted) *inserted = 0;
+    if (c->cfg.nucleation != NICR_NUC_KSP) return NICR_OK;   // 2D:545
+    NucScratch S{};
+    int rc = noalloc_scratch(c, &S);
+    if (rc) return rc;
+    NucPar N{0, 0, 0, c->cfg.radius_grid, c->cfg.exclusion_gap_grid, 4096};
+    if (c->dim == 2) {
+        if (c->te == 8) launch_nucleate<2, double>(c, N, S, true);
+        else launch_nucleate<2, float>(c, N, S, true);
+    } else {
+        if (c->te == 8) launch_nucleate<3, double>(c, N, S, true);
+        else launch_nucleate<3, float>(c, N, S, true);
+    }
+    HIPCHK(hipGetLastError());
+    int n = 0;
+    HIPCHK(hipMemcpyAsync(&n, S.misc + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (n && c->has_eta()) c->mask_dirty = true;
     c->refresh_valid = false;
