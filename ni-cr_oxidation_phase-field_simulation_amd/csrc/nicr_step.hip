@@ -922,23 +922,42 @@ struct EtaCell {
     float coef, fdtl, K, W2;    // binary32
     V S2, pr;                   // running sums entering this pass
 };
+// Per-cell arrays of the eta kernels are addressed by the cell's byte offset
+// cb = cell * sizeof(TE) as a 32-bit value (the launcher guarantees that the
+// fields of one pass, and so every cell offset, stay below 2^32 bytes): a
+// scalar base plus one 32-bit lane offset instead of a 64-bit address per array.
+template <class T>
+__device__ __forceinline__ T ld_off(const T* base, unsigned ob) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + ob);
+}
+template <class T>
+__device__ __forceinline__ void st_nt_off(T* base, unsigned ob, T v) {
+    st_nt(reinterpret_cast<T*>(reinterpret_cast<char*>(base) + ob), v);
+}
 template <int PREC, class TE, class TM>
-__device__ __forceinline__ EtaCell<PREC, TE> eta_cell_coefs(const Par& P, const Bufs<TE, TM>& B, long long c,
+__device__ __forceinline__ EtaCell<PREC, TE> eta_cell_coefs(const Par& P, const Bufs<TE, TM>& B, unsigned cb,
                                                             int cur_i, int nxt_i, bool first_pass) {
     using V = typename EtaCell<PREC, TE>::V;
     EtaCell<PREC, TE> e;
     if constexpr (PREC == PREC_FP64) {
-        e.pp = B.p[cur_i][c];
-        e.S = (double)B.s2[cur_i][c];
+        static_assert(sizeof(TE) == sizeof(double), "binary64 eta: phi and the mask share the cell offset");
+        e.pp = ld_off(B.p[cur_i], cb);
+        e.S = (double)ld_off(B.s2[cur_i], cb);
         e.dtl = P.dt * P.etaL;
     } else {
-        e.coef = B.ecoef ? B.ecoef[c] : eta_coef32(P, (float)B.s2[cur_i][c], (float)B.p[cur_i][c]);
+        static_assert(sizeof(TE) == sizeof(float), "mixed eta: the force factor shares the cell offset");
+        if (B.ecoef) {
+            e.coef = ld_off(B.ecoef, cb);
+        } else {
+            const long long c = (long long)(cb / sizeof(TE));
+            e.coef = eta_coef32(P, (float)B.s2[cur_i][c], (float)B.p[cur_i][c]);
+        }
         e.fdtl = (float)(P.dt * P.etaL);
         e.K = (float)P.etaK;
         e.W2 = 2.f * (float)P.etaW;
     }
-    e.S2 = first_pass ? (V)0 : (V)B.s2[nxt_i][c];
-    e.pr = first_pass ? (V)0 : (V)B.pairs[nxt_i][c];
+    e.S2 = first_pass ? (V)0 : (V)ld_off(B.s2[nxt_i], cb);
+    e.pr = first_pass ? (V)0 : (V)ld_off(B.pairs[nxt_i], cb);
     return e;
 }
 template <int PREC, class TE>
@@ -973,7 +992,7 @@ eta_value(const Par& P, const EtaCell<PREC, TE>& e, const TE* L, int oym, int oy
 template <int PREC, int NG, class TE, class TM>
 __device__ __forceinline__ void eta_cell_update(const Par& P, const Bufs<TE, TM>& B, const EtaCell<PREC, TE>& e,
                                                 const TE* lds, int PL, int oym, int oyp, int ozm, int ozp,
-                                                const TE* prev, const TE* cv, const TE* nxv, long long c, int nxt_i,
+                                                const TE* prev, const TE* cv, const TE* nxv, unsigned cbo, int nxt_i,
                                                 bool last_pass, long long key0, __amdgpu_buffer_rsrc_t rout,
                                                 unsigned fs, unsigned long long& bad, bool active) {
     using T = TE;
@@ -983,7 +1002,8 @@ __device__ __forceinline__ void eta_cell_update(const Par& P, const Bufs<TE, TM>
     // values are not the trial) stores through an offset beyond the buffer
     // resource's range, which the hardware discards, so the common path has no
     // divergent branch around the field loop.
-    const unsigned cb = active ? (unsigned)c * (unsigned)sizeof(T) : 0xFFFFFFF0u;
+    const unsigned cb = active ? cbo : 0xFFFFFFF0u;
+    const long long c = (long long)(cbo / sizeof(T));   // the cell, for the report key
     V S2 = e.S2, pr = e.pr;
     bool ok = true;
 #pragma unroll
@@ -1014,6 +1034,10 @@ __device__ __forceinline__ void eta_cell_update(const Par& P, const Bufs<TE, TM>
         // once (register pressure): groups of four fields
         if ((g & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
+    // Materialise the running sums before the slow path: otherwise the compiler
+    // sinks their chain below it, which keeps all NG trial values live across
+    // the slow path and pushes the kernel past its register budget.
+    if constexpr (PREC != PREC_FP64) asm volatile("" : "+v"(S2), "+v"(pr));
     if (!active) return;
     if (PREC != PREC_FP64 && !ok) {
         for (int g = 0; g < NG; ++g) {
@@ -1025,11 +1049,11 @@ __device__ __forceinline__ void eta_cell_update(const Par& P, const Bufs<TE, TM>
             }
         }
     }
-    st_nt(&B.s2[nxt_i][c], (TE)S2);
-    if (!last_pass) st_nt(&B.pairs[nxt_i][c], (TE)pr);   // the next pass's running sum
+    st_nt_off(B.s2[nxt_i], cbo, (TE)S2);
+    if (!last_pass) st_nt_off(B.pairs[nxt_i], cbo, (TE)pr);   // the next pass's running sum
     if (last_pass) {
         const V s16 = (V)16 * pr;
-        st_nt(&B.gb[nxt_i][c], (TE)((V)P.eta_scale * (s16 < (V)1 ? s16 : (V)1)));
+        st_nt_off(B.gb[nxt_i], cbo, (TE)((V)P.eta_scale * (s16 < (V)1 ? s16 : (V)1)));
     }
 }
 
@@ -1147,11 +1171,11 @@ __global__ __launch_bounds__(64 * TY) void k_eta_march(Par P, Bufs<TE, TM> B, in
                 lds[g * PL + ly * W + lz] = hv[u];
             }
         }
-        const EtaCell<PREC, TE> ec = eta_cell_coefs<PREC>(P, B, xo + rc, cur_i, nxt_i, first_pass);
+        const EtaCell<PREC, TE> ec = eta_cell_coefs<PREC>(P, B, cell_bytes(xo + rc), cur_i, nxt_i, first_pass);
         __syncthreads();
         eta_cell_update<PREC, NG>(P, B, ec, lds, PL, ty * W + tz + 1, (ty + 2) * W + tz + 1, (ty + 1) * W + tz,
-                                  (ty + 1) * W + tz + 2, prev, cv, nx_, xo + rc, nxt_i, last_pass, key0, rout, fs,
-                                  bad, active);
+                                  (ty + 1) * W + tz + 2, prev, cv, nx_, cell_bytes(xo + rc), nxt_i, last_pass, key0,
+                                  rout, fs, bad, active);
         __syncthreads();
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
@@ -1210,9 +1234,18 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(eta_rows_wa
     const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc(
         B.eta[nxt_i] + (long long)g0 * P.fstride, 0, (int)(fs * (unsigned)NG), BUF_DWORD3);
     auto cell_bytes = [](long long cell) { return (unsigned)cell * (unsigned)sizeof(T); };
-    // halo work of this thread: pairs ty + k*TY = (halo row: bit 0, field: the rest)
-    const long long hrow0 = (long long)(((y0 - 1) % P.ny + P.ny) % P.ny) * nz + z;
-    const long long hrow1 = (long long)((y0 + TY) % P.ny) * nz + z;
+    // Cell byte offsets are 32-bit (below 2^32, see above): the own cell's
+    // offset within a plane, and a plane's offset, uniform over the block.
+    const unsigned rcb = cell_bytes(rc);
+    auto plane_bytes = [&](int X) { return (unsigned)__builtin_amdgcn_readfirstlane((int)cell_bytes((long long)X * plane)); };
+    // halo work of this thread: pairs ty + k*TY = (halo row: bit 0, field: the
+    // rest). TY is even, so the row is ty & 1 for every k (one byte offset and
+    // one LDS row per thread) and the field is (ty >> 1) + k * TY / 2.
+    static_assert(TY % 2 == 0, "the halo row of a thread must not depend on k");
+    const long long hrow = (ty & 1) ? (long long)((y0 + TY) % P.ny) * nz + z
+                                    : (long long)(((y0 - 1) % P.ny + P.ny) % P.ny) * nz + z;
+    const unsigned hrow_b = cell_bytes(hrow);
+    const int hlds = ((ty & 1) ? (TY + 1) * nz : 0) + z;
     // LDS offsets of the cell's Y/Z neighbours within a field tile
     const int own = (ty + 1) * nz + z;
     const int oym = own - nz, oyp = own + nz;
@@ -1224,8 +1257,8 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(eta_rows_wa
         const int xp = x0 == 0 ? 0 : x0 - 1;
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            prev[g] = buf_load<T>(rin, cell_bytes(xp * plane + rc), (unsigned)g * fs);
-            cv[g] = buf_load<T>(rin, cell_bytes((long long)x0 * plane + rc), (unsigned)g * fs);
+            prev[g] = buf_load<T>(rin, plane_bytes(xp) + rcb, (unsigned)g * fs);
+            cv[g] = buf_load<T>(rin, plane_bytes(x0) + rcb, (unsigned)g * fs);
         }
     }
     // Halo rows of plane X (a wave lies inside one row, as nz is a multiple of
@@ -1239,44 +1272,41 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(eta_rows_wa
     constexpr bool AHEAD = PREC == PREC_MIXED;
     T hv[HC];
     auto load_halo = [&](int X) {
-        const long long xo = (long long)X * plane;
+        const unsigned vo = plane_bytes(X) + hrow_b;
 #pragma unroll
         for (int k = 0; k < HC; ++k) {
-            const int pr = __builtin_amdgcn_readfirstlane(ty + k * TY), g = pr >> 1;
-            hv[k] = buf_load<T>(rin, cell_bytes(xo + ((pr & 1) ? hrow1 : hrow0)), (unsigned)g * fs);
+            const int g = __builtin_amdgcn_readfirstlane((ty >> 1) + k * (TY / 2));
+            hv[k] = buf_load<T>(rin, vo, (unsigned)g * fs);
         }
     };
     if (AHEAD) load_halo(x0);
     unsigned long long bad = NO_KEY;
     for (int x = x0; x < x1; ++x) {
-        const long long xo = (long long)x * plane;
+        const unsigned cb = plane_bytes(x) + rcb;   // the own cell of plane x
         const int xq = x == P.nx - 1 ? x : x + 1;
-        const long long xn = (long long)xq * plane;
+        const unsigned nb = plane_bytes(xq) + rcb;  // ... of plane xq
         // binary32: the cell's own inputs of plane x, first of this iteration's loads
         EtaCell<PREC, TE> ec;
-        if (AHEAD) ec = eta_cell_coefs<PREC>(P, B, xo + rc, cur_i, nxt_i, first_pass);
+        if (AHEAD) ec = eta_cell_coefs<PREC>(P, B, cb, cur_i, nxt_i, first_pass);
         if (!AHEAD) {
             load_halo(x);
 #pragma unroll
-            for (int g = 0; g < NG; ++g) nx_[g] = buf_load<T>(rin, cell_bytes(xn + rc), (unsigned)g * fs);
+            for (int g = 0; g < NG; ++g) nx_[g] = buf_load<T>(rin, nb, (unsigned)g * fs);
         }
         // plane x (interior and halo) into LDS, then the next plane's interior
         // and halo rows into the freed registers
 #pragma unroll
         for (int g = 0; g < NG; ++g) lds[g * PL + own] = cv[g];
 #pragma unroll
-        for (int k = 0; k < HC; ++k) {
-            const int pr = ty + k * TY, g = pr >> 1;
-            lds[g * PL + ((pr & 1) ? (TY + 1) * nz : 0) + z] = hv[k];
-        }
+        for (int k = 0; k < HC; ++k) lds[((ty >> 1) + k * (TY / 2)) * PL + hlds] = hv[k];
         if (AHEAD) {
 #pragma unroll
-            for (int g = 0; g < NG; ++g) nx_[g] = buf_load<T>(rin, cell_bytes(xn + rc), (unsigned)g * fs);
+            for (int g = 0; g < NG; ++g) nx_[g] = buf_load<T>(rin, nb, (unsigned)g * fs);
             if (x + 1 < x1) load_halo(xq);
         }
         __syncthreads();
-        if (!AHEAD) ec = eta_cell_coefs<PREC>(P, B, xo + rc, cur_i, nxt_i, first_pass);
-        eta_cell_update<PREC, NG>(P, B, ec, lds, PL, oym, oyp, ozm, ozp, prev, cv, nx_, xo + rc, nxt_i, last_pass,
+        if (!AHEAD) ec = eta_cell_coefs<PREC>(P, B, cb, cur_i, nxt_i, first_pass);
+        eta_cell_update<PREC, NG>(P, B, ec, lds, PL, oym, oyp, ozm, ozp, prev, cv, nx_, cb, nxt_i, last_pass,
                                   key0, rout, fs, bad, active);
         __syncthreads();
 #pragma unroll
