@@ -212,126 +212,143 @@ def unet_flops_per_cell():
     return sum(2 * 27 * ci * co / 8 ** lv for lv, convs in levels.items() for ci, co in convs)
 
 
-def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, warm, tol, seed, n_eta, n_seeds, settle, noise):
-    """Learned surrogate on the closed workload: training set from n_traj
-    plain binary64 trajectories at n_train^3, Adam on the GPU, then n_run^3 from
-    one state (after `warm` plain steps) advanced nsteps by the plain binary64
-    step and by surrogate jumps of 50 steps (in logged chunks of 511 steps);
-    both timed, the surrogate's final fields and O/Cr totals compared with the
-    plain run's.
-    n_eta > 0: n_seeds grains at n_run^3 coloured onto n_eta evolving eta fields
-    (training volumes hold the same grain size); n_eta = 0: static grains.
+def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, tol, seed, settle, noise, runs, fallback):
+    """Learned surrogate on the inlet workload: a training set from n_traj plain
+    binary64 trajectories of the static-grain inlet generator at n_train^3
+    (reference Voronoi grains of --grain-diameter-grid 63, Cr 0.20 +- 0.005,
+    O inlet on x = 0, legacy Ck, Ksp scans on the reference schedule), Adam on
+    the GPU; then at n_run^3, for each workload in `runs`, nsteps of the
+    reference loop with its Ksp schedule by the plain binary64 step and with
+    50-step network jumps, both timed; the surrogate's final fields, totals,
+    budgets and budget closure compared with the plain run's.
+    runs: "evolving" (32 evolving eta from 64 Voronoi seeds, the headline mesh's
+    state) and/or "static" (the generator's static grains at n_run^3).
     settle: extra plain steps between a jump's check step and the next proposal;
+    fallback: plain steps after a rejected jump;
     noise: largest relative perturbation of the recorded samples."""
     sur = importlib.import_module(PKG_NAME + ".surrogate")
-    syn = importlib.import_module(PKG_NAME + ".synthetic")
+    wl = _wl()
 
     def log(msg):
         print(f"[bench] surrogate: {msg}", file=sys.stderr, flush=True)
 
     J = 50
-    train_seeds = max(2, round(n_seeds * (n_train / n_run) ** 3))
+    # a proposal may undershoot O slightly ahead of the inlet front, where the
+    # plain step itself clips (legacy bounds); the check step clips those
+    # values and records them, so only values far outside reject a jump
+    bounds = (-1e-2, 1.0)
     net = sur.UNet((n_train,) * 3, max_batch=2, train=True, max_samples=n_traj, seed=seed)
-    rec = sur.record_trajectories(net, nicr, n_traj, jump=J, n_seeds=train_seeds, seed=seed, log=log, n_eta=n_eta,
-                                  noise=noise)
+    rec = sur.record_inlet_trajectories(net, nicr, n_traj, jump=J, seed=seed, log=log, noise=noise)
     net.fit_norm()
+    net.set_augment(True)   # samples through random Y/Z shifts and mirrors (symmetries of the step)
     tr = sur.train_schedule(net, iters, batch=2, lr=1e-3, seed=seed, log=log)
     params, norm = net.params(), net.norm()
     net.close()
     shape = (n_run,) * 3
     cells = n_run ** 3
-    t0 = time.perf_counter()
-    state = syn.closed_alloy(shape, n_seeds=n_seeds, seed=seed + 7, n_eta=n_eta)
-    log(f"{n_run}^3 state built in {time.perf_counter() - t0:.1f} s")
-    plain = sur.closed_model(nicr, shape, state)
-    sm = sur.closed_model(nicr, shape, state)
-    del state
-    for m in (plain, sm):
-        r = m.step(warm)
-        if r.steps_done != warm:
-            raise RuntimeError("surrogate workload: warm-up step rejected")
     inet = sur.UNet(shape, seed=1)
     inet.set_params(params)
     inet.set_norm(norm)
     inet.set_timing(True)
-    mass0 = [sum(sm.stats(f)[0]) for f in ("o", "cr")]
-    plain.sync()
-    t0 = time.perf_counter()
-    r = plain.step(nsteps)
-    plain.sync()
-    t_plain = time.perf_counter() - t0
-    log(f"plain binary64: {nsteps} steps in {t_plain:.2f} s")
-    tot = {"steps": 0, "acc": 0, "rej": 0, "plain": 0, "res": [0.0, 0.0, 0.0], "min_rej": None}
-    chunk = 1 + 10 * (J + 1)   # one plain step, then ten jumps with their check steps
-    t_sur = 0.0
-    while tot["steps"] < nsteps:
-        k = min(chunk, nsteps - tot["steps"])
+    znet = sur.UNet(shape, seed=1)   # an untrained network's head is zero
+    znet.set_norm(norm)
+    out = {"training": {"generator": f"{n_train}^3 static Voronoi grains (--grain-diameter-grid 63), Cr 0.20+-0.005, "
+                                     "O inlet on x=0 (concentration BC), Ck legacy, bounds legacy, Ksp scans",
+                        "volume": f"{n_train}^3", "trajectories": rec["trajectories"], "noise": noise,
+                        "augment": "random Y/Z shifts and mirrors",
+                        "plain_steps": rec["plain_steps"], "record_s": round(rec["seconds"], 1),
+                        "iters": tr["iters"], "batch": tr["batch"], "train_s": round(tr["seconds"], 1),
+                        "loss_first": round(tr["loss_first"], 4), "loss_last": round(tr["loss_last"], 4)},
+           "jump": J, "residual_tol": tol, "settle_steps": settle, "fallback_steps": fallback,
+           "bounds": list(bounds), "runs": {}}
+    for kind in runs:
+        t0 = time.perf_counter()
+        if kind == "evolving":
+            state = wl.evolving_grain_state(n_run)
+            make = lambda: wl.evolving_grain_model("fp64", state)
+            n_eta = len(state[3])
+            desc = f"3D {n_run}^3, {n_eta} evolving eta from 64 Voronoi seeds"
+        else:
+            gb = wl.static_grain_gb(shape, 63.0, wl.SEED)
+            fields = importlib.import_module(PKG_NAME + ".synthetic").noisy_alloy(shape, 0.20, 0.005, seed=wl.SEED)
+            make = lambda: wl.inlet_model(nicr, "fp64", fields, gb=gb)
+            n_eta = 0
+            desc = f"3D {n_run}^3, static Voronoi grains (--grain-diameter-grid 63)"
+        plain, sm = make(), make()
+        log(f"{kind}: {n_run}^3 models built in {time.perf_counter() - t0:.1f} s")
+        t_start, b_start = wl.totals(sm), sm.b
+        plain.sync()
+        t0 = time.perf_counter()
+        rp = plain.run_loop(nsteps)
+        plain.sync()
+        t_plain = time.perf_counter() - t0
+        log(f"{kind}: plain binary64: {nsteps} steps in {t_plain:.2f} s, {rp['scans']} scans")
         sm.sync()
         t0 = time.perf_counter()
-        rep = inet.advance(sm, k, jump=J, residual_tol=tol, settle_steps=settle)
+        rs = inet.run_loop(sm, nsteps, jump=J, residual_tol=tol, settle_steps=settle, bounds=bounds,
+                           fallback_steps=fallback)
         sm.sync()
-        dt_chunk = time.perf_counter() - t0
-        t_sur += dt_chunk
-        if rep.steps_done != k or rep.status != 0:
-            raise RuntimeError("surrogate workload: a plain step was rejected")
-        tot["steps"] += k
-        tot["acc"] += rep.jumps_accepted
-        tot["rej"] += rep.jumps_rejected
-        tot["plain"] += rep.plain_steps
-        tot["res"] = [max(a, b) for a, b in zip(tot["res"], (rep.max_residual_o, rep.max_residual_cr,
-                                                             rep.max_residual_eta))]
-        if rep.jumps_rejected:
-            mr = rep.min_rejected_residual
-            tot["min_rej"] = mr if tot["min_rej"] is None else min(tot["min_rej"], mr)
-        log(f"steps {tot['steps'] - k}-{tot['steps']}: {rep.jumps_accepted} jumps accepted, {rep.jumps_rejected} "
-            f"rejected (smallest rejected residual {rep.min_rejected_residual:.3f}), {dt_chunk * 1e3:.1f} ms")
-    if r.steps_done != nsteps:
-        raise RuntimeError("surrogate workload: a plain step was rejected")
-    err = {}
-    for f in ("o", "cr", "p"):
-        a, b = sm.get(f), plain.get(f)
-        nb = np.linalg.norm(b)
-        err[f] = float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.abs(a - b).max())
-    if n_eta:
-        num = den = 0.0
-        for g in range(n_eta):
-            a, b = sm.get(g), plain.get(g)
-            num += float(np.sum((a - b) ** 2))
-            den += float(np.sum(b * b))
-        err["eta"] = (num / den) ** 0.5
-    mass_sur = [sum(sm.stats(f)[0]) for f in ("o", "cr")]
-    mass_plain = [sum(plain.stats(f)[0]) for f in ("o", "cr")]
-    fwd_ms = inet.forward_ms()
-    flops = unet_flops_per_cell() * cells
-    dt = plain.c.dt
-    out = {
-        "workload": f"3D {n_run}^3 closed, " + (f"{n_eta} evolving eta from {n_seeds} Voronoi seeds"
-                                                  if n_eta else f"{n_seeds} static Voronoi grains (GB mask)")
-                    + f", Cr 0.20+-0.005, O 1e-3 (+-20%), no oxide; binary64 steps; jump {J} steps",
-        "speedup": round(t_plain / t_sur, 3),
-        "sim_time_per_s_plain_fp64": nsteps * dt / t_plain,
-        "sim_time_per_s_surrogate": nsteps * dt / t_sur,
-        "steps": nsteps, "warm_steps": warm, "ms_plain": round(t_plain * 1e3, 2),
-        "ms_surrogate": round(t_sur * 1e3, 2),
-        "jumps_accepted": tot["acc"], "jumps_rejected": tot["rej"], "plain_steps": tot["plain"],
-        "residual_tol": tol, "settle_steps": settle, "max_residual": [round(tot["res"][0], 4), round(tot["res"][1], 4)]
-                                             + ([float(f"{tot['res'][2]:.3e}")] if n_eta else []),
-        "min_rejected_residual": None if tot["min_rej"] is None else round(tot["min_rej"], 4),
-        "rel_l2_vs_plain": {k: float(f"{v:.3e}") for k, v in err.items()},
-        "mass_rel_vs_plain": [float(f"{abs(a - b) / abs(b):.3e}") for a, b in zip(mass_sur, mass_plain)],
-        "mass_rel_vs_start": [float(f"{abs(a - b) / abs(b):.3e}") for a, b in zip(mass_sur, mass0)],
-        "forward_ms": round(fwd_ms, 3),
-        "forward_roofline": {"bound": "mfma", "achieved": round(flops / (fwd_ms * 1e-3) / 1e12, 1), "peak": 2500.0,
-                             "unit": "TFLOP/s", "frac": round(flops / (fwd_ms * 1e-3) / 1e12 / 2500.0, 4),
-                             "flops_per_forward": flops},
-        "training": {"volume": f"{n_train}^3", "trajectories": rec["trajectories"], "noise": noise,
-                     "plain_steps": rec["plain_steps"], "record_s": round(rec["seconds"], 1),
-                     "iters": tr["iters"], "batch": tr["batch"], "train_s": round(tr["seconds"], 1),
-                     "loss_first": round(tr["loss_first"], 4), "loss_last": round(tr["loss_last"], 4)},
-    }
+        t_sur = time.perf_counter() - t0
+        log(f"{kind}: surrogate: {nsteps} steps in {t_sur:.2f} s: {rs['jumps_accepted']} jumps accepted, "
+            f"{rs['jumps_rejected']} rejected, {rs['plain_steps']} plain steps, {rs['scans']} scans")
+        if rp["steps"] != nsteps or rs["steps"] != nsteps or rs["status"] != 0:
+            raise RuntimeError(f"surrogate workload {kind}: a plain step was rejected")
+        # the same checked jumps without the network: a zero head proposes x0 + J u0
+        zm = make()
+        zm.sync()
+        t0 = time.perf_counter()
+        rz = znet.run_loop(zm, nsteps, jump=J, residual_tol=tol, settle_steps=settle, bounds=bounds,
+                           fallback_steps=fallback)
+        zm.sync()
+        t_zero = time.perf_counter() - t0
+        log(f"{kind}: extrapolation only: {nsteps} steps in {t_zero:.2f} s: {rz['jumps_accepted']} jumps accepted, "
+            f"{rz['jumps_rejected']} rejected")
+        err_z = wl.field_errors(zm, plain)
+        zero_line = {"speedup": round(t_plain / t_zero, 3), "ms": round(t_zero * 1e3, 1),
+                     "jumps_accepted": rz["jumps_accepted"], "jumps_rejected": rz["jumps_rejected"],
+                     "max_residual": [float(f"{v:.3e}") for v in rz["max_residual"][:3 if n_eta else 2]],
+                     "rel_l2_vs_plain": {k: float(f"{v['rel_l2']:.3e}") for k, v in err_z.items()}}
+        zm.close()
+        err = wl.field_errors(sm, plain)
+        tsur, tpl = wl.totals(sm), wl.totals(plain)
+        bs, bp = sm.b, plain.b
+        srcs = {sp: {k: (bs[k + sp] - b_start[k + sp], bp[k + sp] - b_start[k + sp]) for k in ("bc", "ck", "clip", "nuc")}
+                for sp in ("O", "Cr")}
+        dt = plain.c.dt
+        fwd_ms = inet.forward_ms()
+        flops = unet_flops_per_cell() * cells
+        out["runs"][kind] = {
+            "workload": desc + ", Cr 0.20+-0.005, O inlet on x=0, Ck legacy, bounds legacy, Ksp scans on the "
+                               f"reference schedule, binary64 steps; {nsteps} steps from the initial state",
+            "speedup": round(t_plain / t_sur, 3),
+            "sim_time_per_s_plain_fp64": nsteps * dt / t_plain,
+            "sim_time_per_s_surrogate": nsteps * dt / t_sur,
+            "ms_plain": round(t_plain * 1e3, 1), "ms_surrogate": round(t_sur * 1e3, 1),
+            "scans": rs["scans"], "inserted_cells": [rs["inserted"], rp["inserted"]],
+            "jumps_accepted": rs["jumps_accepted"], "jumps_rejected": rs["jumps_rejected"],
+            "plain_steps": rs["plain_steps"],
+            "max_residual": [float(f"{v:.3e}") for v in rs["max_residual"][:3 if n_eta else 2]],
+            "min_rejected_residual": rs["min_rejected_residual"],
+            "rel_l2_vs_plain": {k: float(f"{v['rel_l2']:.3e}") for k, v in err.items()},
+            "rel_linf_vs_plain": {k: float(f"{v['rel_linf']:.3e}") for k, v in err.items()},
+            "totals_rel_vs_plain": {f: float(f"{abs(tsur[f] - tpl[f]) / abs(tpl[f]):.3e}") for f in tsur},
+            "closure_surrogate": {k: float(f"{v:.3e}") for k, v in wl.closure(sm, t_start).items()},
+            "closure_plain": {k: float(f"{v:.3e}") for k, v in wl.closure(plain, t_start).items()},
+            "sources_surrogate_vs_plain": {sp: {k: [float(f"{a:.6e}"), float(f"{b:.6e}")] for k, (a, b) in d.items()}
+                                           for sp, d in srcs.items()},
+            "extrapolation_only": zero_line,
+            "forward_ms": round(fwd_ms, 3),
+            "forward_roofline": {"bound": "mfma", "achieved": round(flops / (fwd_ms * 1e-3) / 1e12, 1),
+                                 "peak": 2500.0, "unit": "TFLOP/s",
+                                 "frac": round(flops / (fwd_ms * 1e-3) / 1e12 / 2500.0, 4), "flops_per_forward": flops},
+        }
+        plain.close()
+        sm.close()
     inet.close()
-    plain.close()
-    sm.close()
+    znet.close()
+    first = out["runs"][runs[0]]
+    out["speedup"] = first["speedup"]
+    out["workload"] = runs[0]
     return out
 
 
@@ -357,14 +374,14 @@ def main():
     ap.add_argument("--sur-train-n", type=int, default=128, help="side of the training volumes")
     ap.add_argument("--sur-run-n", type=int, default=256, help="side of the surrogate run")
     ap.add_argument("--sur-steps", type=int, default=5000, help="steps of the timed run (5000: the 256^3 run length)")
-    ap.add_argument("--sur-warm", type=int, default=0, help="plain steps before the timed run")
-    ap.add_argument("--sur-settle", type=int, default=0,
+    ap.add_argument("--sur-settle", type=int, default=2,
                     help="extra plain steps between a jump's check step and the next proposal")
-    ap.add_argument("--sur-noise", type=float, default=0.4,
+    ap.add_argument("--sur-fallback", type=int, default=50, help="plain steps after a rejected jump")
+    ap.add_argument("--sur-noise", type=float, default=0.0,
                     help="largest perturbation of the recorded training states (x jump x RMS update)")
     ap.add_argument("--sur-tol", type=float, default=0.1, help="trapezoid residual tolerance of a jump")
-    ap.add_argument("--sur-static", action="store_true",
-                    help="surrogate on static grains (128 seeds at 256^3) instead of the --eta evolving fields")
+    ap.add_argument("--sur-runs", default="evolving,static",
+                    help="surrogate workloads at --sur-run-n: evolving (32 eta) and/or static grains, comma-separated")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the 1 GiB device copy reference (rocprof counter calibration)")
     args = ap.parse_args()
@@ -381,9 +398,8 @@ def main():
         return
     def run_surrogate():
         return surrogate_bench(nicr, args.sur_train_n, args.sur_run_n, args.sur_traj, args.sur_iters,
-                               args.sur_steps, args.sur_warm, args.sur_tol, 20260909, 0 if args.sur_static else args.eta,
-                               128 * (args.sur_run_n // 256) ** 3 if args.sur_static else args.seeds,
-                               args.sur_settle, args.sur_noise)
+                               args.sur_steps, args.sur_tol, 20260909, args.sur_settle, args.sur_noise,
+                               tuple(args.sur_runs.split(",")), args.sur_fallback)
 
     if args.surrogate_only:
         print(json.dumps({"surrogate": run_surrogate()}), flush=True)

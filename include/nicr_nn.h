@@ -89,6 +89,11 @@ int nicr_sur_samples(void* net, int* count);
  * deviations, means and standard deviations of the volume statistics, gains
  * that give the normalised corrections unit mean square, and the jump. */
 int nicr_sur_fit_norm(void* net);
+/* Training samples through the symmetries of the step (1) or as recorded (0,
+ * the default): each drawn sample is shifted periodically in Y and Z by random
+ * offsets and mirrored in Y and in Z with probability 1/2 each; X, which holds
+ * the inlet, is kept. No counterpart in the reference. */
+int nicr_sur_set_augment(void* net, int on);
 /* `iters` Adam iterations (learning rate lr) on random batches of `batch`
  * samples drawn with `seed`; the gradient is clipped to 3 times the running
  * mean of its global norm, and an iteration with a non-finite gradient leaves
@@ -101,7 +106,7 @@ int nicr_sur_eval(void* net, int first, int count, double* mse);
 /* ---- surrogate-accelerated stepping ---- */
 typedef struct nicr_sur_options {
     int32_t jump;             /* steps replaced by one network proposal (50) */
-    int32_t conserve_mass;    /* rescale O and Cr to their pre-jump totals (closed boundaries) */
+    int32_t conserve_mass;    /* O and Cr totals after a jump follow its sources (see nicr_sur_advance) */
     double residual_tol;      /* accept when ||R|| <= residual_tol * ||proposal - x0|| for O and Cr */
     double min_value, max_value;   /* accept only O, Cr, phi inside [min, max] */
     int32_t settle_steps;     /* extra plain steps before a proposal that follows other steps (0: none) */
@@ -130,7 +135,7 @@ typedef struct nicr_sur_report {
  * xh = x0 + jump u0 + correction(x0, u0) of O, Cr, phi for the state `jump`
  * steps later (jump must equal the network's); evolving eta fields jump to
  * x0 + jump u0 with their mask rebuilt, static ones are unchanged; optionally
- * O and Cr rescaled to their totals at x0; one plain check step from xh to
+ * O and Cr rescaled to their totals (below); one plain check step from xh to
  * xh1. The jump is accepted when O, Cr, phi (and evolving eta) are inside
  * [min, max], the model accepts the check step, and for O, Cr (and all
  * evolving eta fields together) the trapezoid residual
@@ -138,9 +143,16 @@ typedef struct nicr_sur_report {
  * ||R|| <= residual_tol ||xh - x0|| (an O or Cr proposal that does not move
  * fails). Otherwise the model is put back exactly to x0 (fields, eta, mask,
  * state index, budgets) and takes fallback_steps (default `jump`) plain steps
- * instead. Only models whose plain step takes no O or Cr from outside can jump:
- * a model with an open O boundary (bc concentration or mu with O diffusion on),
- * legacy Ck or Ksp nucleation is refused with NICR_E_INVALID and left unchanged. */
+ * instead.
+ * Sources over a jump: the budget terms of the `jump` steps it replaces (O
+ * inlet of the concentration or mu boundary, legacy Ck, clipping) are the
+ * trapezoid jump/2 (s_pre + s_check) of the per-step sources of the step that
+ * gave u0 and of the check step; they are added to the budgets and the clipped
+ * count. With conserve_mass, O and Cr are rescaled to their totals at x0 plus
+ * jump s_pre before the check step, and after it by the trapezoid's
+ * difference, so the totals follow the budgets (a closed box: the totals at
+ * x0). Ksp scans are not run (as nicr_step runs none): the caller ends each
+ * call at the next scan and calls nicr_nucleate there. */
 int nicr_sur_advance(void* net, void* model, int nsteps, const nicr_sur_options* opt, nicr_sur_report* report);
 
 #ifdef __cplusplus

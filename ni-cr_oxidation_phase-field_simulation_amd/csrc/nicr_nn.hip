@@ -169,15 +169,54 @@ __global__ void k_input(int V, const T0* __restrict__ f0, const T0* __restrict__
     dst[2] = dst[3] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+// A symmetry of the step applied to a training sample: Y and Z are periodic
+// and the stencils are mirror-symmetric, so a sample shifted by (sy, sz) and
+// mirrored in Y (fy) or Z (fz) is again a trajectory sample of the step; X,
+// which holds the inlet, is kept. Position p of the transformed volume reads
+// cell src(p) of the recorded one.
+struct SampleMap {
+    int ny, nz, sy, sz, fy, fz;
+    __device__ __forceinline__ int src(int p) const {
+        const int nyz = ny * nz, x = p / nyz, r = p - x * nyz, y = r / nz, z = r - y * nz;
+        int ys = (fy ? ny - 1 - y : y) + sy, zs = (fz ? nz - 1 - z : z) + sz;
+        ys -= ys >= ny ? ny : 0;
+        zs -= zs >= nz ? nz : 0;
+        return x * nyz + ys * nz + zs;
+    }
+};
+
+// Network input of one recorded (binary32) training volume through a map.
+__global__ void k_input_map(int V, const float* __restrict__ in, SampleMap m, VolNorm vn, bf16* __restrict__ x0) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= V) return;
+    const int q = m.src(p);
+    bf16x8 v, w;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = (bf16)(float)(((double)in[(long long)c * V + q] - vn.mean[c]) * vn.inv[c]);
+#pragma unroll
+    for (int c = 0; c < N_OUT; ++c) v[4 + c] = (bf16)(float)((double)in[(long long)(4 + c) * V + q] * vn.uinv[c]);
+    v[7] = (bf16)vn.cst[0];
+    w[0] = (bf16)vn.cst[1];
+    w[1] = (bf16)vn.cst[2];
+    w[2] = (bf16)vn.cst[3];
+#pragma unroll
+    for (int i = 3; i < 8; ++i) w[i] = (bf16)0.f;
+    uint4* dst = reinterpret_cast<uint4*>(x0 + 32LL * p);
+    dst[0] = __builtin_bit_cast(uint4, v);
+    dst[1] = __builtin_bit_cast(uint4, w);
+    dst[2] = dst[3] = make_uint4(0u, 0u, 0u, 0u);
+}
+
 // Training target of one volume: tgt[p][c] = (d_c - J u_c) * out_inv_c, row pitch 4.
-__global__ void k_target(int V, const float* __restrict__ d, const float* __restrict__ u, VolNorm vn,
+__global__ void k_target(int V, const float* __restrict__ d, const float* __restrict__ u, SampleMap m, VolNorm vn,
                          float* __restrict__ tgt) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= V) return;
+    const int q = m.src(p);
     float t[N_OUT];
 #pragma unroll
     for (int c = 0; c < N_OUT; ++c)
-        t[c] = (float)(((double)d[(long long)c * V + p] - vn.jump * (double)u[(long long)c * V + p]) * vn.out_inv[c]);
+        t[c] = (float)(((double)d[(long long)c * V + q] - vn.jump * (double)u[(long long)c * V + q]) * vn.out_inv[c]);
     *reinterpret_cast<float4*>(tgt + 4LL * p) = make_float4(t[0], t[1], t[2], 0.f);
 }
 
@@ -535,6 +574,7 @@ inline unsigned nblk(long long n, int t = 256) { return (unsigned)((n + t - 1) /
 struct Net {
     int nbmax = 1, S[3] = {0, 0, 0};
     bool train = false;
+    bool augment = false;   // training samples through random Y/Z shifts and mirrors
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
     size_t bytes = 0;
@@ -987,17 +1027,19 @@ int device_stats(Net* n, const double* o, const double* cr, double* const u[N_OU
     return NICR_OK;
 }
 
-// network input and target of batch slot `slot` from training sample `si`
-void load_sample(Net* n, int slot, int si, bool target) {
+// network input and target of batch slot `slot` from training sample `si`,
+// through the symmetry map m (the identity: the sample as recorded)
+void load_sample(Net* n, int slot, int si, bool target, SampleMap m) {
     const long long V = n->V(0);
     const float* in = n->ds_in + (long long)si * N_IN * V;
     const VolNorm vn = vol_norm(n->norm, &n->ds_stat[N_STAT * (size_t)si]);
-    hipLaunchKernelGGL((k_input<float, float, float>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, in, in + V,
-                       in + 2 * V, in + 3 * V, in + 4 * V, in + 5 * V, in + 6 * V, vn, n->x0 + (long long)slot * V * 32);
+    hipLaunchKernelGGL(k_input_map, dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, in, m, vn,
+                       n->x0 + (long long)slot * V * 32);
     if (target)
         hipLaunchKernelGGL(k_target, dim3(nblk(V)), dim3(256), 0, n->stream, (int)V,
-                           n->ds_d + (long long)si * N_OUT * V, in + 4 * V, vn, n->tgt + 4LL * slot * V);
+                           n->ds_d + (long long)si * N_OUT * V, in + 4 * V, m, vn, n->tgt + 4LL * slot * V);
 }
+SampleMap identity_map(const Net* n) { return SampleMap{n->S[1], n->S[2], 0, 0, 0, 0}; }
 
 int check_ctx_shape(Net* n, Ctx* c) {
     if (c->dim != 3 || c->P.nx != n->S[0] || c->P.ny != n->S[1] || c->P.nz != n->S[2])
@@ -1442,6 +1484,13 @@ int nicr_sur_fit_norm(void* h) {
 // `iters` Adam iterations on random batches of `batch` training samples. The
 // per-iteration losses (mean squared error of the normalised deltas) go to
 // losses[iters] when given.
+int nicr_sur_set_augment(void* h, int on) {
+    Net* n = static_cast<Net*>(h);
+    if (!n) return fail(NICR_E_INVALID, "nicr_sur_set_augment: null network");
+    n->augment = on != 0;
+    return NICR_OK;
+}
+
 int nicr_sur_train(void* h, int iters, int batch, float lr, uint64_t seed, float* losses) {
     Net* n = static_cast<Net*>(h);
     if (!n || iters < 0) return fail(NICR_E_INVALID, "nicr_sur_train: bad argument");
@@ -1459,8 +1508,19 @@ int nicr_sur_train(void* h, int iters, int batch, float lr, uint64_t seed, float
     HIPCHK(hipMemsetAsync(n->loss_acc, 0, (size_t)iters * 4, n->stream));
     std::mt19937_64 rng(seed);
     std::uniform_int_distribution<int> pick(0, n->ds_n - 1);
+    std::uniform_int_distribution<int> shy(0, n->S[1] - 1), shz(0, n->S[2] - 1), flip(0, 1);
     for (int it = 0; it < iters; ++it) {
-        for (int b = 0; b < batch; ++b) load_sample(n, b, pick(rng), true);
+        for (int b = 0; b < batch; ++b) {
+            const int si = pick(rng);
+            SampleMap m = identity_map(n);
+            if (n->augment) {
+                m.sy = shy(rng);
+                m.sz = shz(rng);
+                m.fy = flip(rng);
+                m.fz = flip(rng);
+            }
+            load_sample(n, b, si, true, m);
+        }
         forward(n, batch);
         HIPCHK(hipMemsetAsync(n->grad, 0, n->nparam * 4, n->stream));
         backward(n, batch, n->loss_acc + it);
@@ -1486,7 +1546,7 @@ int nicr_sur_eval(void* h, int first, int count, double* mse) {
     if (!n->train) return fail(NICR_E_STATE, "nicr_sur_eval: network created without training buffers");
     HIPCHK(hipMemsetAsync(n->dacc, 0, 64 * 8, n->stream));
     for (int s = first; s < first + count; ++s) {
-        load_sample(n, 0, s, true);
+        load_sample(n, 0, s, true, identity_map(n));
         forward(n, 1);
         HIPCHK(hipMemsetAsync(n->grad, 0, n->nparam * 4, n->stream));
         backward(n, 1, reinterpret_cast<float*>(n->dacc));
@@ -1589,18 +1649,13 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
     if (!n || !c || !opt || !rep || nsteps < 0) return fail(NICR_E_INVALID, "nicr_sur_advance: bad argument");
     int rc = check_ctx_shape(n, c);
     if (rc) return rc;
-    // A jump moves the state by a network proposal and, with conserve_mass,
-    // rescales O and Cr to their pre-jump totals. That is the model's own
-    // evolution only where the plain step adds no O or Cr from outside: no
-    // inlet (an open O boundary feeds bcO every step, 2D:408-411,435-442), no
-    // legacy Ck source (2D:567-571) and no Ksp scans falling due inside a jump
-    // (2D:1222). Any other model is refused rather than advanced wrongly.
-    if (c->cfg.oxygen_diffusion && c->cfg.bc != NICR_BC_CLOSED)
-        return fail(NICR_E_INVALID, "nicr_sur_advance: the model has an open O boundary (bc must be closed)");
-    if (c->cfg.ck == NICR_CK_LEGACY)
-        return fail(NICR_E_INVALID, "nicr_sur_advance: legacy Ck adds O and Cr every step (ck must be off)");
-    if (c->cfg.nucleation == NICR_NUC_KSP)
-        return fail(NICR_E_INVALID, "nicr_sur_advance: Ksp scans would fall inside jumps (nucleation must not be ksp)");
+    // A jump replaces `jump` plain steps, and with them those steps' O and Cr
+    // sources: the inlet (2D:408-411,435-442), legacy Ck (2D:567-571) and
+    // clipping (2D:574-580). They are carried over the jump by the trapezoid
+    // rule from the step before it and the check step after it, into the
+    // budgets and, with conserve_mass, into the O and Cr totals. Ksp scans are
+    // not run here: like nicr_step, a call never reaches one, and the caller
+    // ends each call at the next scan (2D:1222).
     const int J = opt->jump;
     if (J < 2) return fail(NICR_E_INVALID, "nicr_sur_advance: jump must be >= 2");
     if (J != n->norm.jump) return fail(NICR_E_INVALID, "nicr_sur_advance: jump differs from the network's");
@@ -1629,6 +1684,34 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         }
         return NICR_OK;
     };
+    // Per-step sources: the change of each budget slot over one step (slots
+    // nucO, nucCr, ckO, ckCr, clipO, clipCr, bcO, bcCr; even slots are O) and of
+    // the clipped-cell count. src_last holds those of the last step taken,
+    // which is also the step that gives u0.
+    constexpr int NSLOT = 8;
+    auto budget_at = [](const DevCtl& h, int k) { return (long double)h.budget[2 * k] + (long double)h.budget[2 * k + 1]; };
+    auto sources = [&](const DevCtl& a, const DevCtl& b, long double* d) {
+        for (int k = 0; k < NSLOT; ++k) d[k] = budget_at(b, k) - budget_at(a, k);
+        d[NSLOT] = (long double)(b.clipped - a.clipped);
+    };
+    auto species_sum = [&](const long double* d, int sp) {
+        long double t = 0.L;
+        for (int k = sp; k < NSLOT; k += 2) t += d[k];
+        return t;
+    };
+    long double src_last[NSLOT + 1] = {};
+    // plain steps whose last one is followed by a jump: its sources are kept
+    auto plain_tracked = [&](int k) -> int {
+        if (k <= 0) return NICR_OK;
+        int e = plain(k - 1);
+        if (e || rep->status != NICR_STEP_OK) return e;
+        DevCtl a, b;
+        if ((e = sync_ctl(c, &a))) return e;
+        if ((e = plain(1))) return e;
+        if ((e = sync_ctl(c, &b))) return e;
+        sources(a, b, src_last);
+        return NICR_OK;
+    };
     if (opt->settle_steps < 0) return fail(NICR_E_INVALID, "nicr_sur_advance: settle_steps must be >= 0");
     if (opt->fallback_steps < 0) return fail(NICR_E_INVALID, "nicr_sur_advance: fallback_steps must be >= 0");
     const int fallback = opt->fallback_steps > 0 ? opt->fallback_steps : J;
@@ -1643,7 +1726,7 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
             break;
         }
         // plain steps before the jump (none when the last step already gives u0)
-        if ((rc = plain(pre))) return rc;
+        if ((rc = plain_tracked(pre))) return rc;
         if (rep->status != NICR_STEP_OK) break;
         DevCtl saved;
         if ((rc = sync_ctl(c, &saved))) return rc;   // also: the step has finished
@@ -1675,12 +1758,16 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
             }
         }
         c->refresh_valid = false;
-        // O and Cr back to their totals at x0 (closed boundaries conserve both)
+        // O and Cr to their totals at x0 plus `jump` steps at the sources of
+        // the step before the jump (a closed box without clipping: the totals
+        // at x0); the check step's sources refine this below
         double f[2] = {1., 1.};
         const double s1[2] = {(double)pa[0], (double)pa[1]};
+        const double target[2] = {(double)((long double)m0[0] + (long double)J * species_sum(src_last, 0)),
+                                  (double)((long double)m0[1] + (long double)J * species_sum(src_last, 1))};
         if (conserve) {
             for (int k = 0; k < 2; ++k)
-                if (s1[k] > 0. && m0[k] > 0.) f[k] = m0[k] / s1[k];
+                if (s1[k] > 0. && target[k] > 0.) f[k] = target[k] / s1[k];
             hipLaunchKernelGGL(k_scale2, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], f[0], c->cr[cur], f[1]);
             HIPCHK(hipStreamSynchronize(n->stream));
         }
@@ -1734,6 +1821,48 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         rep->last_residual[1] = rcr;
         rep->last_residual[2] = reta;
         if (ok) {
+            // the jump's sources by the trapezoid rule over its `jump` steps,
+            // into the budgets; with conserve_mass the state after the check
+            // step takes the difference to the first-order target above, so
+            // the totals stay consistent with the budgets
+            DevCtl after;
+            if ((rc = sync_ctl(c, &after))) return rc;
+            long double src_check[NSLOT + 1];
+            sources(saved, after, src_check);
+            long double add[NSLOT + 1];
+            for (int k = 0; k <= NSLOT; ++k) add[k] = 0.5L * (long double)J * (src_last[k] + src_check[k]);
+            bool changed = false;
+            for (int k = 0; k < NSLOT; ++k) {
+                if (add[k] == 0.L) continue;
+                const long double v = budget_at(after, k) + add[k];
+                after.budget[2 * k] = (double)v;
+                after.budget[2 * k + 1] = (double)(v - (long double)after.budget[2 * k]);
+                changed = true;
+            }
+            const long long nclip = (long long)std::llround((double)add[NSLOT]);
+            if (nclip) {
+                after.clipped += nclip;
+                changed = true;
+            }
+            if (changed) {
+                HIPCHK(hipMemcpyAsync(c->ctl, &after, sizeof(after), hipMemcpyHostToDevice, c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+            }
+            if (conserve) {
+                double g[2] = {1., 1.};
+                for (int sp = 0; sp < 2; ++sp) {
+                    const long double now = (long double)target[sp] + species_sum(src_check, sp);
+                    const long double want = (long double)m0[sp] + species_sum(add, sp) + species_sum(src_check, sp);
+                    if (now > 0.L && want > 0.L && want != now) g[sp] = (double)(want / now);
+                }
+                if (g[0] != 1. || g[1] != 1.) {
+                    const int k = c->cur;
+                    hipLaunchKernelGGL(k_scale2, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[k], g[0], c->cr[k],
+                                       g[1]);
+                    HIPCHK(hipStreamSynchronize(n->stream));
+                }
+            }
+            for (int k = 0; k <= NSLOT; ++k) src_last[k] = src_check[k];
             ++rep->jumps_accepted;
             rep->max_residual_o = std::max(rep->max_residual_o, ro);
             rep->max_residual_cr = std::max(rep->max_residual_cr, rcr);
@@ -1767,7 +1896,7 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         }
         HIPCHK(hipStreamSynchronize(n->stream));
         c->refresh_valid = false;
-        if ((rc = plain(std::min(fallback, left)))) return rc;
+        if ((rc = plain_tracked(std::min(fallback, left)))) return rc;
         consecutive = true;
     }
     rep->steps_done = nsteps - left;

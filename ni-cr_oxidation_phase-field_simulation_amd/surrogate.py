@@ -82,6 +82,7 @@ def _declare(lib):
         "nicr_sur_samples": [_VOIDP, ctypes.POINTER(i)],
         "nicr_sur_fit_norm": [_VOIDP],
         "nicr_sur_train": [_VOIDP, i, i, f, u64, _FPTR],
+        "nicr_sur_set_augment": [_VOIDP, i],
         "nicr_sur_eval": [_VOIDP, i, i, _DPTR],
         "nicr_sur_advance": [_VOIDP, _VOIDP, i, ctypes.POINTER(SurOptions), ctypes.POINTER(SurReport)],
     }
@@ -215,6 +216,10 @@ class UNet:
     def fit_norm(self) -> None:
         _check(self.lib, self.lib.nicr_sur_fit_norm(self.h))
 
+    def set_augment(self, on: bool) -> None:
+        """Train on samples through random Y/Z shifts and mirrors (nicr_sur_set_augment)."""
+        _check(self.lib, self.lib.nicr_sur_set_augment(self.h, int(bool(on))))
+
     def train(self, iters: int, batch: int = 1, lr: float = 1e-3, seed: int = 1) -> np.ndarray:
         losses = np.zeros(max(1, iters), np.float32)
         _check(self.lib, self.lib.nicr_sur_train(self.h, int(iters), int(batch), float(lr), int(seed), _fp(losses)))
@@ -235,6 +240,52 @@ class UNet:
         _check(self.lib, self.lib.nicr_sur_advance(self.h, model._h.h, int(nsteps), ctypes.byref(opt),
                                                    ctypes.byref(rep)))
         return rep
+
+    def run_loop(self, model, nsteps: int, first: int = 1, next_event=None, jump: int = 50, **kw) -> dict:
+        """Steps first .. first + nsteps - 1 of the reference run loop
+        (mpi2d/NiCr2D_grains.cpp:1216-1223, mpi3d-slab/NiCr3D_grains.cpp:1365-1372)
+        with network jumps: Model::nucleate runs before step n whenever
+        n dt + 1e-12 period >= the next event time (as Model.run_loop), and the
+        steps between two scans are one advance() call, so no jump crosses a
+        scan. kw: advance()'s options. Returns totals over the calls:
+        {"steps", "scans", "inserted", "next_event", "plain_steps",
+        "jumps_accepted", "jumps_rejected", "max_residual" (O, Cr, eta),
+        "min_rejected_residual", "status"}."""
+        from . import NUCLEATION
+        c = model.c
+        period = c.nucleation_period
+        nxt = period if next_event is None else next_event
+        ksp = c.nucleation == NUCLEATION["ksp"]
+        out = {"steps": 0, "scans": 0, "inserted": 0, "plain_steps": 0, "jumps_accepted": 0, "jumps_rejected": 0,
+               "max_residual": [0.0, 0.0, 0.0], "min_rejected_residual": None, "status": 0}
+        n, last = first, first + nsteps - 1
+        while n <= last:
+            if ksp and n * c.dt + 1.e-12 * period >= nxt:
+                out["inserted"] += model.nucleate()
+                out["scans"] += 1
+                nxt += period
+            m = n
+            while ksp and m < last and (m + 1) * c.dt + 1.e-12 * period < nxt:
+                m += 1
+            if not ksp:
+                m = last
+            rep = self.advance(model, m - n + 1, jump=jump, **kw)
+            out["steps"] += rep.steps_done
+            out["plain_steps"] += rep.plain_steps
+            out["jumps_accepted"] += rep.jumps_accepted
+            out["jumps_rejected"] += rep.jumps_rejected
+            out["max_residual"] = [max(a, b) for a, b in zip(out["max_residual"], (
+                rep.max_residual_o, rep.max_residual_cr, rep.max_residual_eta))]
+            if rep.jumps_rejected:
+                mr = rep.min_rejected_residual
+                out["min_rejected_residual"] = mr if out["min_rejected_residual"] is None else min(
+                    out["min_rejected_residual"], mr)
+            if rep.status != 0:
+                out["status"] = rep.status
+                break
+            n = m + 1
+        out["next_event"] = nxt
+        return out
 
     def set_timing(self, on: bool) -> None:
         _check(self.lib, self.lib.nicr_unet_set_timing(self.h, int(bool(on))))
@@ -276,6 +327,55 @@ def closed_model(nicr, shape, state, precision="fp64"):
     else:
         m.set("gb", grains)
     return m
+
+
+def record_inlet_trajectories(net, nicr, n_traj, jump=50, n_struct=16, diameter=63.0, warm=(20, 5000), seed=0,
+                              log=None, noise=0.0, noise_frac=0.75):
+    """One training sample from each of n_traj plain binary64 trajectories of
+    the static-grain inlet workload (the generator of the 128^3 static-grain
+    configuration): the reference's Voronoi grains of --grain-diameter-grid
+    `diameter` (static GB mask), Cr 0.20 +- U(0.005), O = 0 with the
+    concentration-BC inlet on x = 0, legacy Ck, legacy bounds, Ksp scans on the
+    reference schedule. Trajectory t uses grain structure t % n_struct and its
+    own composition noise, runs a random number of steps in `warm` of the
+    reference loop, then records (state, plain update, change over `jump`
+    steps). noise: as record_trajectories."""
+    from . import workloads as wl
+    from . import synthetic as syn
+    shape = net.shape
+    t0 = time.perf_counter()
+    gbs = [wl.static_grain_gb(shape, diameter, seed + 100 + s) for s in range(n_struct)]
+    t1 = time.perf_counter()
+    steps = 0
+    for t in range(n_traj):
+        r = np.random.default_rng(seed + 1000 + t)
+        o, cr, p = syn.noisy_alloy(shape, 0.20, 0.005, seed=seed + 1000 + t)
+        m = wl.inlet_model(nicr, "fp64", (o, cr, p), gb=gbs[t % n_struct])
+        w = int(r.integers(warm[0], warm[1]))
+        res = m.run_loop(w)
+        if res["steps"] != w:
+            raise RuntimeError(f"trajectory {t}: a plain step was rejected")
+        if noise > 0.0 and r.uniform() < noise_frac:
+            c = r.uniform(0.0, noise)
+            before = {k: m.get(k) for k in ("o", "cr")}
+            m.step(1)
+            steps += 1
+            for k in ("o", "cr"):
+                a = m.get(k)
+                amp = c * jump * float(np.sqrt(np.mean((a - before[k]) ** 2)))
+                m.set(k, np.clip(a + amp * r.standard_normal(shape), 0.0, 1.0))
+        rr = net.record(m, jump)
+        if rr.status != 0:
+            raise RuntimeError(f"trajectory {t}: a plain step was rejected (status {rr.status})")
+        steps += w + 1 + jump
+        m.close()
+        if log and (t + 1) % 50 == 0 and t + 1 < n_traj:
+            log(f"{t + 1} trajectories recorded ({time.perf_counter() - t1:.1f} s)")
+    t2 = time.perf_counter()
+    if log:
+        log(f"{n_struct} grain structures in {t1 - t0:.1f} s; {n_traj} trajectories ({steps} plain steps) "
+            f"recorded in {t2 - t1:.1f} s")
+    return {"trajectories": n_traj, "plain_steps": steps, "seconds": t2 - t0}
 
 
 def record_trajectories(net, nicr, n_traj, jump=50, n_struct=16, n_seeds=16, warm=(20, 600), seed=0, log=None,

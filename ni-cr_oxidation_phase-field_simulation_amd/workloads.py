@@ -25,12 +25,13 @@ from . import synthetic as syn
 SEED = 20260909
 
 
-def static_grain_model(precision: str, n: int = 128, diameter: float = 63.0, seed: int = SEED) -> Model:
-    """The 3D static-grain workload on the device."""
+def static_grain_gb(shape, diameter: float = 63.0, seed: int = SEED) -> np.ndarray:
+    """The GB mask of the reference's static Voronoi grains (--grain-diameter-grid
+    `diameter`, --random-seed `seed`) on a 3D mesh `shape` (host array)."""
     from .host import HostState
-    cfg = make_config(3, nx=n, ny=n, nz=n, grain_diameter_grid=diameter, random_seed=seed, grains=True,
-                      grain_evolution=False, bc="concentration", ck="legacy", bounds="legacy", nucleation="ksp",
-                      precision=precision)
+    nx, ny, nz = shape
+    cfg = make_config(3, nx=nx, ny=ny, nz=nz, grain_diameter_grid=diameter, random_seed=seed, grains=True,
+                      grain_evolution=False, bc="concentration", ck="legacy", bounds="legacy", nucleation="ksp")
     old = os.environ.get("NICR_HOST_INIT")
     os.environ["NICR_HOST_INIT"] = "device"   # the grain distances on the GPU (bitwise the host loops)
     try:
@@ -42,13 +43,37 @@ def static_grain_model(precision: str, n: int = 128, diameter: float = 63.0, see
             os.environ["NICR_HOST_INIT"] = old
     gb = hs.get("gb")
     hs.close()
-    o, cr, p = syn.noisy_alloy((n, n, n), 0.20, 0.005, seed=seed)
+    return gb
+
+
+def inlet_model(nicr_pkg, precision: str, fields, gb=None, eta=None) -> Model:
+    """A 3D model of the inlet workloads holding fields = (o, cr, p): the
+    concentration-BC O inlet on x = 0, legacy Ck, legacy bounds, Ksp scans;
+    static grains with mask gb, or len(eta) evolving eta fields."""
+    o, cr, p = fields
+    nx, ny, nz = o.shape
+    if eta is not None:
+        cfg = make_config(3, nx=nx, ny=ny, nz=nz, grain_evolution=True, n_eta=len(eta), bc="concentration",
+                          bounds="legacy", ck="legacy", nucleation="ksp", precision=precision)
+    else:
+        cfg = make_config(3, nx=nx, ny=ny, nz=nz, grains=True, grain_evolution=False, bc="concentration",
+                          ck="legacy", bounds="legacy", nucleation="ksp", precision=precision)
     m = Model(cfg)
     m.set("o", o)
     m.set("cr", cr)
     m.set("p", p)
-    m.set("gb", gb)
+    if eta is not None:
+        for g, e in enumerate(eta):
+            m.set(g, e)
+    else:
+        m.set("gb", gb)
     return m
+
+
+def static_grain_model(precision: str, n: int = 128, diameter: float = 63.0, seed: int = SEED) -> Model:
+    """The 3D static-grain workload on the device."""
+    gb = static_grain_gb((n, n, n), diameter, seed)
+    return inlet_model(None, precision, syn.noisy_alloy((n, n, n), 0.20, 0.005, seed=seed), gb=gb)
 
 
 def evolving_grain_state(n=256, n_seeds: int = 64, n_eta: int = 32, seed: int = SEED):
@@ -64,16 +89,7 @@ def evolving_grain_state(n=256, n_seeds: int = 64, n_eta: int = 32, seed: int = 
 
 def evolving_grain_model(precision: str, state, n_eta: int = 32) -> Model:
     o, cr, p, eta = state
-    nx, ny, nz = o.shape
-    cfg = make_config(3, nx=nx, ny=ny, nz=nz, grain_evolution=True, n_eta=n_eta, bc="concentration", bounds="legacy",
-                      ck="legacy", nucleation="ksp", precision=precision)
-    m = Model(cfg)
-    m.set("o", o)
-    m.set("cr", cr)
-    m.set("p", p)
-    for g in range(n_eta):
-        m.set(g, eta[g])
-    return m
+    return inlet_model(None, precision, (o, cr, p), eta=[eta[g] for g in range(n_eta)])
 
 
 def totals(m: Model) -> dict:

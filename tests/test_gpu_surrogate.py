@@ -198,26 +198,118 @@ def _plain_norm(net, J):
 
 
 
-@pytest.mark.parametrize("change", [dict(bc="concentration"), dict(bc="mu"), dict(ck="legacy"),
-                                    dict(nucleation="ksp")], ids=["inlet", "mu-inlet", "ck-legacy", "ksp"])
-def test_advance_refuses_models_it_cannot_jump(nicr, change):
-    """A jump is the model's own evolution only without an O inlet, legacy Ck
-    or Ksp scans; any such model is refused with NICR_E_INVALID and left
-    untouched."""
-    shape = (16, 16, 24)
-    state = _syn().closed_alloy(shape, n_seeds=8, seed=4)
-    kw = dict(grains=True, bc="closed", ck="off", bounds="legacy", nucleation="off")
+def _open_model(nicr, shape, seed, **change):
+    state = _syn().closed_alloy(shape, n_seeds=8, seed=seed)
+    kw = dict(grains=True, bc="concentration", ck="legacy", bounds="legacy", nucleation="ksp")
     kw.update(change)
     cfg = nicr.make_config(3, nx=shape[0], ny=shape[1], nz=shape[2], **kw)
     m = nicr.Model(cfg)
     o, cr, p, gb = state
     m.set("o", o); m.set("cr", cr); m.set("p", p); m.set("gb", gb)
+    return m
+
+
+def _totals(m):
+    return {k: sum(m.stats(f)[0]) for k, f in (("O", "o"), ("Cr", "cr"))}
+
+
+def _sources(b, sp):
+    return b["bc" + sp] + b["nuc" + sp] + b["ck" + sp] + b["clip" + sp]
+
+
+def test_jump_sources_are_trapezoid_of_step_sources(nicr):
+    """One accepted jump on the inlet model (O inlet, legacy Ck, Ksp) without
+    mass rescaling and a zero network head: the fields equal the extrapolated
+    state x0 + J (x0 - x_-1) uploaded to a twin and stepped once, bitwise, and
+    the budgets equal the twin's plus J/2 (s_pre + s_check) per slot, s the
+    per-step sources of the step that gave u0 and of the check step."""
+    J = 20
+    shape = (16, 16, 24)
+    a = _open_model(nicr, shape, 5)
+    b = _open_model(nicr, shape, 5)
+    a.step(200)
+    b.step(200)
+    names = ("o", "cr", "p")
+    xm = {f: b.get(f) for f in names}
+    bm = b.b
+    b.step(1)
+    x0 = {f: b.get(f) for f in names}
+    b0 = b.b
+    for f in names:
+        b.set(f, x0[f] + J * (x0[f] - xm[f]))
+    b.step(1)
+    b1 = b.b
+    net = _sur().UNet(shape, seed=1)
+    _plain_norm(net, J)
+    rep = net.advance(a, J + 2, jump=J, conserve_mass=False, residual_tol=1e30, bounds=(-1.0, 2.0))
+    assert rep.status == 0 and rep.steps_done == J + 2 and rep.jumps_accepted == 1 and rep.plain_steps == 2
+    for f in names:
+        assert np.array_equal(a.get(f), b.get(f)), f
+    ba = a.b
+    for k in ("bcO", "ckO", "clipO", "nucO", "bcCr", "ckCr", "clipCr", "nucCr"):
+        s_pre, s_check = b0[k] - bm[k], b1[k] - b0[k]
+        want = b1[k] + 0.5 * J * (s_pre + s_check)
+        assert ba[k] == pytest.approx(want, rel=1e-14, abs=1e-300), k
+    assert b1["bcO"] - b0["bcO"] > 0          # the inlet feeds O every step
+    net.close()
+
+
+@pytest.mark.parametrize("change", [dict(), dict(bc="mu"), dict(bc="closed")],
+                         ids=["inlet-ck-ksp", "mu-inlet", "closed-ck-ksp"])
+def test_open_model_jumps_carry_sources(nicr, change):
+    """Checked jumps on models whose plain step takes O from an inlet
+    (concentration or mu boundary) with legacy Ck and Ksp scans on: with
+    conserve_mass the totals move with the recorded sources, so the reference's
+    diagnostics (change of a total minus the recorded sources, 2D:697-710)
+    close to rounding, and the inlet budget follows plain stepping's within 5 %
+    (extrapolation-only jumps: a zero network head)."""
+    J = 20
+    shape = (16, 16, 24)
+    a = _open_model(nicr, shape, 4, **change)
+    b = _open_model(nicr, shape, 4, **change)
+    a.step(300)
+    b.step(300)
+    t0, b0 = _totals(a), a.b
+    net = _sur().UNet(shape, seed=1)
+    _plain_norm(net, J)
+    rep = net.advance(a, 200, jump=J, residual_tol=0.3)
+    print(f"jumps accepted {rep.jumps_accepted} rejected {rep.jumps_rejected}")
+    assert rep.status == 0 and rep.steps_done == 200
+    assert rep.jumps_accepted >= 1
+    b.step(200)
+    t1, b1 = _totals(a), a.b
+    tb, bb = _totals(b), b.b
+    for sp in ("O", "Cr"):
+        res = (t1[sp] - t0[sp]) - (_sources(b1, sp) - _sources(b0, sp))
+        print(f"{sp}: closure {res / t0['Cr']:.2e}, sources surrogate {_sources(b1, sp) - _sources(b0, sp):.6e} "
+              f"plain {_sources(bb, sp) - _sources(b0, sp):.6e}, total rel vs plain {abs(t1[sp] - tb[sp]) / tb[sp]:.2e}")
+        assert abs(res) <= 1e-12 * t0["Cr"]
+    if change.get("bc", "concentration") != "closed":
+        ds, dp = b1["bcO"] - b0["bcO"], bb["bcO"] - b0["bcO"]
+        assert dp > 0 and abs(ds - dp) <= 0.05 * dp
+    else:
+        assert t1["O"] == pytest.approx(t0["O"], rel=1e-13) and t1["Cr"] == pytest.approx(t0["Cr"], rel=1e-13)
+    net.close()
+
+
+def test_run_loop_scans_on_schedule(nicr):
+    """The surrogate run loop calls Model::nucleate where the reference loop
+    does (2D:1222) and never jumps across a scan: with every jump rejected it
+    equals Model.run_loop bitwise, scans and budgets included."""
+    shape = (16, 16, 24)
+    a = _open_model(nicr, shape, 6)
+    b = _open_model(nicr, shape, 6)
     net = _sur().UNet(shape, seed=1)
     _plain_norm(net, 20)
-    with pytest.raises(RuntimeError, match="nicr_sur_advance"):
-        net.advance(m, 60, jump=20, residual_tol=0.3)
-    assert np.array_equal(m.get("o"), o) and np.array_equal(m.get("cr"), cr)
+    ra = net.run_loop(a, 1100, jump=20, residual_tol=1e-12)
+    rb = b.run_loop(1100)
+    assert ra["status"] == 0 and ra["steps"] == 1100 and ra["jumps_accepted"] == 0 and ra["jumps_rejected"] > 0
+    assert ra["scans"] == rb["scans"] == 2 and ra["inserted"] == rb["inserted"]
+    for f in ("o", "cr", "p"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    assert a.b == b.b
     net.close()
+
 
 @pytest.mark.parametrize("n_eta,settle,fallback", [(0, 0, 0), (8, 0, 0), (8, 2, 7)],
                          ids=["static", "evolving", "evolving-settle2-fallback7"])
