@@ -212,7 +212,8 @@ def unet_flops_per_cell():
     return sum(2 * 27 * ci * co / 8 ** lv for lv, convs in levels.items() for ci, co in convs)
 
 
-def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, tol, seed, settle, noise, runs, fallback):
+def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, tol, seed, settle, noise, runs, fallback,
+                    net_fields=("o", "cr", "phi"), min_jump=0):
     """Learned surrogate on the inlet workload: a training set from n_traj plain
     binary64 trajectories of the static-grain inlet generator at n_train^3
     (reference Voronoi grains of --grain-diameter-grid 63, Cr 0.20 +- 0.005,
@@ -225,7 +226,10 @@ def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, tol, seed, sett
     state) and/or "static" (the generator's static grains at n_run^3).
     settle: extra plain steps between a jump's check step and the next proposal;
     fallback: plain steps after a rejected jump;
-    noise: largest relative perturbation of the recorded samples."""
+    noise: largest relative perturbation of the recorded samples;
+    net_fields: the fields whose jump takes the network's correction (the
+    others jump by the linear extrapolation alone: output gain 0);
+    min_jump: shortest last jump before a scan (0: plain steps there)."""
     sur = importlib.import_module(PKG_NAME + ".surrogate")
     wl = _wl()
 
@@ -248,6 +252,9 @@ def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, tol, seed, sett
     cells = n_run ** 3
     inet = sur.UNet(shape, seed=1)
     inet.set_params(params)
+    for k, f in enumerate(("o", "cr", "phi")):
+        if f not in net_fields:
+            norm.gain[k] = 0.0
     inet.set_norm(norm)
     inet.set_timing(True)
     znet = sur.UNet(shape, seed=1)   # an untrained network's head is zero
@@ -260,6 +267,7 @@ def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, tol, seed, sett
                         "iters": tr["iters"], "batch": tr["batch"], "train_s": round(tr["seconds"], 1),
                         "loss_first": round(tr["loss_first"], 4), "loss_last": round(tr["loss_last"], 4)},
            "jump": J, "residual_tol": tol, "settle_steps": settle, "fallback_steps": fallback,
+           "network_fields": list(net_fields), "min_jump": min_jump,
            "bounds": list(bounds), "runs": {}}
     for kind in runs:
         t0 = time.perf_counter()
@@ -286,7 +294,7 @@ def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, tol, seed, sett
         sm.sync()
         t0 = time.perf_counter()
         rs = inet.run_loop(sm, nsteps, jump=J, residual_tol=tol, settle_steps=settle, bounds=bounds,
-                           fallback_steps=fallback)
+                           fallback_steps=fallback, min_jump=min_jump)
         sm.sync()
         t_sur = time.perf_counter() - t0
         log(f"{kind}: surrogate: {nsteps} steps in {t_sur:.2f} s: {rs['jumps_accepted']} jumps accepted, "
@@ -298,7 +306,7 @@ def surrogate_bench(nicr, n_train, n_run, n_traj, iters, nsteps, tol, seed, sett
         zm.sync()
         t0 = time.perf_counter()
         rz = znet.run_loop(zm, nsteps, jump=J, residual_tol=tol, settle_steps=settle, bounds=bounds,
-                           fallback_steps=fallback)
+                           fallback_steps=fallback, min_jump=min_jump)
         zm.sync()
         t_zero = time.perf_counter() - t0
         log(f"{kind}: extrapolation only: {nsteps} steps in {t_zero:.2f} s: {rz['jumps_accepted']} jumps accepted, "
@@ -377,9 +385,13 @@ def main():
     ap.add_argument("--sur-settle", type=int, default=2,
                     help="extra plain steps between a jump's check step and the next proposal")
     ap.add_argument("--sur-fallback", type=int, default=50, help="plain steps after a rejected jump")
-    ap.add_argument("--sur-noise", type=float, default=0.0,
-                    help="largest perturbation of the recorded training states (x jump x RMS update)")
+    ap.add_argument("--sur-noise", type=float, default=0.2,
+                    help="largest perturbation of the recorded training states' Cr (x jump x RMS update)")
     ap.add_argument("--sur-tol", type=float, default=0.1, help="trapezoid residual tolerance of a jump")
+    ap.add_argument("--sur-min-jump", type=int, default=12,
+                    help="shortest last jump before a Ksp scan (0: the steps left there are plain steps)")
+    ap.add_argument("--sur-net-fields", default="cr,phi",
+                    help="fields whose jump takes the network's correction (others: linear extrapolation)")
     ap.add_argument("--sur-runs", default="evolving,static",
                     help="surrogate workloads at --sur-run-n: evolving (32 eta) and/or static grains, comma-separated")
     ap.add_argument("--copy-probe-only", action="store_true",
@@ -399,7 +411,8 @@ def main():
     def run_surrogate():
         return surrogate_bench(nicr, args.sur_train_n, args.sur_run_n, args.sur_traj, args.sur_iters,
                                args.sur_steps, args.sur_tol, 20260909, args.sur_settle, args.sur_noise,
-                               tuple(args.sur_runs.split(",")), args.sur_fallback)
+                               tuple(args.sur_runs.split(",")), args.sur_fallback,
+                               tuple(args.sur_net_fields.split(",")), args.sur_min_jump)
 
     if args.surrogate_only:
         print(json.dumps({"surrogate": run_surrogate()}), flush=True)

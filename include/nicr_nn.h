@@ -50,12 +50,18 @@ int nicr_unet_get_grads(void* net, float* host);
  * divisor is 0); 7..10 constant fields (c_k - c_mean[k]) / c_scale[k] of c =
  * (log10 s_O, m_Cr, log10 r_O, log10 r_Cr); channels 11..31 are zero. The
  * network predicts the correction to the linear extrapolation of the update
- * over `jump` steps: change of field k = jump u_k + out_k jump r_k gain[k]. */
+ * over `jump` steps: change of field k = jump u_k + out_k jump r_k gain[k].
+ * stat_planes > 0 takes m, s and r over the first stat_planes X planes only
+ * (the inlet side), so that a mesh deeper than the training volumes presents
+ * the statistics of a training volume's depth (an inlet front fills a smaller
+ * share of a deeper mesh); 0 or >= nx: the whole volume. nicr_sur_fit_norm
+ * sets it to the training volumes' s0. */
 typedef struct nicr_unet_norm {
     float phi_mean, phi_scale, gb_mean, gb_scale;
     float c_mean[4], c_scale[4];
     float gain[3];
     int32_t jump;
+    int32_t stat_planes;
 } nicr_unet_norm;
 int nicr_unet_set_norm(void* net, const nicr_unet_norm* norm);
 int nicr_unet_get_norm(void* net, nicr_unet_norm* norm);
@@ -111,6 +117,7 @@ typedef struct nicr_sur_options {
     double min_value, max_value;   /* accept only O, Cr, phi inside [min, max] */
     int32_t settle_steps;     /* extra plain steps before a proposal that follows other steps (0: none) */
     int32_t fallback_steps;   /* plain steps after a rejected jump before the next proposal (0: jump) */
+    int32_t min_jump;         /* shortest jump at the end of a call (0: full jumps only; see nicr_sur_advance) */
 } nicr_sur_options;
 
 typedef struct nicr_sur_report {
@@ -152,7 +159,13 @@ typedef struct nicr_sur_report {
  * jump s_pre before the check step, and after it by the trapezoid's
  * difference, so the totals follow the budgets (a closed box: the totals at
  * x0). Ksp scans are not run (as nicr_step runs none): the caller ends each
- * call at the next scan and calls nicr_nucleate there. */
+ * call at the next scan and calls nicr_nucleate there.
+ * Near the end of a call, fewer steps than a full jump (plus its check step)
+ * may be left: with min_jump in [2, jump) a last jump of Jc >= min_jump steps
+ * takes them, with the linear extrapolation Jc u0 and the network's
+ * correction scaled by (Jc / jump)^2 (the deviation from a linear path is
+ * second order in the jump length), checked like any other; otherwise, and
+ * when it is rejected, plain steps take them. */
 int nicr_sur_advance(void* net, void* model, int nsteps, const nicr_sur_options* opt, nicr_sur_report* report);
 
 #ifdef __cplusplus

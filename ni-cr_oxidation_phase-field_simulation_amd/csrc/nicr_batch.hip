@@ -332,23 +332,30 @@ BBufs bbufs(BatchCtx* b) {
     return B;
 }
 
+// Resident blocks of k_batch_pde<TX> on the device (its persistent grid), or 0
+// when a device query fails: a launch then takes one block per tile, which is
+// always a valid grid. A failed LDS attribute surfaces as the launch's error.
 template <int TX>
 int batch_resident(BatchCtx* b) {
-    static int resident = 0;
-    if (!resident) {
+    static int resident = -1;
+    if (resident < 0) {
         const size_t lds = BatchLds<TX>::bytes();
-        (void)hipFuncSetAttribute((const void*)k_batch_pde<TX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         int per_cu = 0, ncu = 0, dev = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_pde<TX>, 64 * TX, lds);
-        resident = std::max(1, per_cu) * std::max(1, ncu);
+        const bool ok =
+            hipFuncSetAttribute((const void*)k_batch_pde<TX>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) ==
+                hipSuccess &&
+            hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_pde<TX>, 64 * TX, lds) == hipSuccess &&
+            per_cu > 0 && ncu > 0;
+        resident = ok ? per_cu * ncu : 0;
     }
     return resident;
 }
 template <int TX>
 void batch_pde_launch(BatchCtx* b, const BBufs& B, int ntile) {
-    const int grid = std::max(1, std::min(ntile, b->grid > 0 ? b->grid : batch_resident<TX>(b)));
+    const int res = batch_resident<TX>(b);
+    const int grid = std::max(1, std::min(ntile, b->grid > 0 ? b->grid : (res > 0 ? res : ntile)));
     hipLaunchKernelGGL(k_batch_pde<TX>, dim3(grid), dim3(64 * TX), BatchLds<TX>::bytes(), b->stream, b->P, B, ntile);
 }
 // one ensemble step (dry: discarded, for tuning)

@@ -35,6 +35,72 @@ constexpr int N_OUT = 3;              // changes of O, Cr, phi
 constexpr int N_IN = N_FIELD + N_OUT; // stored per volume: the fields and the plain update of O, Cr, phi
 constexpr int N_STAT = 7;             // mean O, std O, mean Cr, std Cr, rms update O, Cr, phi
 
+// ---- launch checks ----
+// Every kernel launch of the network is checked on the host before it is
+// enqueued: a grid of 1 .. 2^31 - 1 blocks in X and 1 in Y/Z, a block of 1 ..
+// the kernel's own maximum threads (its launch bounds), and static plus
+// dynamic LDS within the 160 KiB of a CU. A launch that fails a check is not
+// enqueued; the first failure is kept (per host thread) and the entry point
+// that made the launch returns it as NICR_E_INVALID.
+constexpr size_t LDS_PER_CU = 160 * 1024;
+struct LaunchCheck {
+    bool failed = false;
+    std::string msg;
+};
+thread_local LaunchCheck g_launch;
+
+struct KernelLimits {
+    const void* fn;
+    int max_threads;
+    size_t static_lds;
+};
+
+bool launch_ok(const void* fn, dim3 grid, dim3 block, size_t dyn_lds, const char* name) {
+    static thread_local std::vector<KernelLimits> cache;
+    const KernelLimits* kl = nullptr;
+    for (const auto& k : cache)
+        if (k.fn == fn) kl = &k;
+    if (!kl) {
+        hipFuncAttributes a{};
+        if (hipFuncGetAttributes(&a, fn) != hipSuccess) {
+            if (!g_launch.failed) {
+                g_launch.failed = true;
+                g_launch.msg = std::string("launch check: no attributes for ") + name;
+            }
+            return false;
+        }
+        cache.push_back({fn, a.maxThreadsPerBlock, a.sharedSizeBytes});
+        kl = &cache.back();
+    }
+    const unsigned long long threads = (unsigned long long)block.x * block.y * block.z;
+    const char* why = nullptr;
+    if (grid.x < 1 || grid.x > 0x7fffffffu || grid.y != 1 || grid.z != 1) why = "grid";
+    else if (threads < 1 || threads > (unsigned long long)kl->max_threads) why = "block";
+    else if (kl->static_lds + dyn_lds > LDS_PER_CU) why = "LDS";
+    if (!why) return true;
+    if (!g_launch.failed) {
+        g_launch.failed = true;
+        g_launch.msg = std::string("launch check: ") + name + ": " + why + " out of range (grid " +
+                       std::to_string(grid.x) + "x" + std::to_string(grid.y) + "x" + std::to_string(grid.z) +
+                       ", block " + std::to_string(threads) + " of max " + std::to_string(kl->max_threads) +
+                       ", LDS " + std::to_string(kl->static_lds) + " + " + std::to_string(dyn_lds) + " B)";
+    }
+    return false;
+}
+
+// the sticky launch failure of this thread as the entry point's result (cleared)
+int launch_status() {
+    if (!g_launch.failed) return NICR_OK;
+    g_launch.failed = false;
+    return fail(NICR_E_INVALID, g_launch.msg);
+}
+
+#define NN_LAUNCH(fn, grid, block, lds, stream, ...)                                                  \
+    do {                                                                                              \
+        if (::nicr::nn::launch_ok((const void*)(fn), dim3(grid), dim3(block), (size_t)(lds), #fn))    \
+            hipLaunchKernelGGL(fn, dim3(grid), dim3(block), lds, stream, __VA_ARGS__);                \
+    } while (0)
+
 // ---- head, loss, inputs, optimiser ----
 
 // out[p][c] = bh[c] + sum_ci D[p][ci] Wh[c][ci] (c < 3; out row pitch 4).
@@ -693,10 +759,10 @@ void launch_conv(Net* n, const Geo& g, const bf16* X, const bf16* W, const float
         const int upb = (units + nblock - 1) / nblock;
         const int grid = (units + upb - 1) / upb;
         if (relu)
-            hipLaunchKernelGGL(k_conv_halo<true>, dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W, bias,
+            NN_LAUNCH(k_conv_halo<true>, dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W, bias,
                                Y, units, upb, Xc);
         else
-            hipLaunchKernelGGL(k_conv_halo<false>, dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W,
+            NN_LAUNCH(k_conv_halo<false>, dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W,
                                bias, Y, units, upb, Xc);
         return;
     }
@@ -704,12 +770,12 @@ void launch_conv(Net* n, const Geo& g, const bf16* X, const bf16* W, const float
     const unsigned mb = (unsigned)((P + CBM - 1) / CBM);
     if (g.cout % 64 == 0) {
         const dim3 grid(mb * (g.cout / 64));
-        if (relu) hipLaunchKernelGGL((k_conv<64, true>), grid, dim3(256), 0, n->stream, g, X, W, bias, Y);
-        else hipLaunchKernelGGL((k_conv<64, false>), grid, dim3(256), 0, n->stream, g, X, W, bias, Y);
+        if (relu) NN_LAUNCH((k_conv<64, true>), grid, dim3(256), 0, n->stream, g, X, W, bias, Y);
+        else NN_LAUNCH((k_conv<64, false>), grid, dim3(256), 0, n->stream, g, X, W, bias, Y);
     } else {
         const dim3 grid(mb * (g.cout / 32));
-        if (relu) hipLaunchKernelGGL((k_conv<32, true>), grid, dim3(256), 0, n->stream, g, X, W, bias, Y);
-        else hipLaunchKernelGGL((k_conv<32, false>), grid, dim3(256), 0, n->stream, g, X, W, bias, Y);
+        if (relu) NN_LAUNCH((k_conv<32, true>), grid, dim3(256), 0, n->stream, g, X, W, bias, Y);
+        else NN_LAUNCH((k_conv<32, false>), grid, dim3(256), 0, n->stream, g, X, W, bias, Y);
     }
 }
 
@@ -720,7 +786,7 @@ void launch_wgrad(Net* n, const Geo& g, const bf16* X, const bf16* dY, float* dW
     int chunk = (P + nchunk - 1) / nchunk;
     chunk = std::max(128, (chunk + 127) / 128 * 128);
     nchunk = (P + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_conv_wgrad, dim3((unsigned)(tiles * g.taps * nchunk)), dim3(256), 0, n->stream, g, X, dY, dW,
+    NN_LAUNCH(k_conv_wgrad, dim3((unsigned)(tiles * g.taps * nchunk)), dim3(256), 0, n->stream, g, X, dY, dW,
                        chunk);
 }
 
@@ -729,7 +795,7 @@ void launch_bias_grad(Net* n, int P, int C, const bf16* G, float* dB) {
     int nchunk = std::max(1, std::min(2048 / cg, (P + 1023) / 1024));
     const int chunk = (P + nchunk - 1) / nchunk;
     nchunk = (P + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)(cg * nchunk)), dim3(256), 0, n->stream, P, C, chunk, G, dB);
+    NN_LAUNCH(k_bias_grad, dim3((unsigned)(cg * nchunk)), dim3(256), 0, n->stream, P, C, chunk, G, dB);
 }
 
 int net_build(Net* n) {
@@ -826,7 +892,7 @@ void cast_weights(Net* n) {
     for (int i = 0; i < N_CONV; ++i) {
         ConvL& L = n->conv[i];
         const long long nw = 27LL * L.cin * L.cout;
-        hipLaunchKernelGGL(k_cast_w, dim3(nblk(nw)), dim3(256), 0, n->stream, 27, L.cout, L.cin, n->param + L.w_off,
+        NN_LAUNCH(k_cast_w, dim3(nblk(nw)), dim3(256), 0, n->stream, 27, L.cout, L.cin, n->param + L.w_off,
                            L.Wb, L.Wt);
     }
 }
@@ -860,12 +926,12 @@ void forward(Net* n, int nb) {
     };
     auto pool = [&](int level, int C, const bf16* X, bf16* Y) {
         const long long nq = (long long)nb * n->V(level + 1);
-        hipLaunchKernelGGL(k_pool, dim3(nblk(nq * (C / 8))), dim3(256), 0, s, (int)nq, n->S[0] >> level,
+        NN_LAUNCH(k_pool, dim3(nblk(nq * (C / 8))), dim3(256), 0, s, (int)nq, n->S[0] >> level,
                            n->S[1] >> level, n->S[2] >> level, C, X, Y);
     };
     auto upcat = [&](int level, int Cu, int Cs, const bf16* L, const bf16* Sk, bf16* U) {
         const long long np = (long long)nb * n->V(level);
-        hipLaunchKernelGGL(k_up_cat, dim3(nblk(np * ((Cu + Cs) / 8))), dim3(256), 0, s, (int)np, n->S[0] >> level,
+        NN_LAUNCH(k_up_cat, dim3(nblk(np * ((Cu + Cs) / 8))), dim3(256), 0, s, (int)np, n->S[0] >> level,
                            n->S[1] >> level, n->S[2] >> level, Cu, Cs, L, Sk, U);
     };
     // decoder convolution i on [up(L), Sk]: without training buffers and where
@@ -900,7 +966,7 @@ void forward(Net* n, int nb) {
     upcat_conv(12, 64, n->d2, n->e1);
     conv(13);
     const long long P0 = (long long)nb * n->V(0);
-    hipLaunchKernelGGL(k_head, dim3(nblk(P0)), dim3(256), 0, s, (int)P0, n->d1, n->param + n->head_w,
+    NN_LAUNCH(k_head, dim3(nblk(P0)), dim3(256), 0, s, (int)P0, n->d1, n->param + n->head_w,
                        n->param + n->head_b, n->out);
     if (n->timing) hipEventRecord(n->ev[1], s);
 }
@@ -911,7 +977,7 @@ void backward(Net* n, int nb, float* loss_slot) {
     hipStream_t s = n->stream;
     ConvL* c = n->conv;
     const long long P0 = (long long)nb * n->V(0);
-    hipLaunchKernelGGL(k_head_bwd, dim3((unsigned)std::min<long long>(1024, nblk(P0))), dim3(HB_THREADS), 0, s,
+    NN_LAUNCH(k_head_bwd, dim3((unsigned)std::min<long long>(1024, nblk(P0))), dim3(HB_THREADS), 0, s,
                        (int)P0, n->d1, n->out, n->tgt, n->param + n->head_w, n->g_d1, n->grad + n->head_w,
                        n->grad + n->head_b, loss_slot);
     // conv i backward: weight and bias gradients from gy (pre-activation
@@ -927,18 +993,18 @@ void backward(Net* n, int nb, float* loss_slot) {
         launch_conv(n, gt, L.gy, L.Wt, nullptr, L.gx, false);
         if (mask_x) {
             const long long n8 = (long long)g.positions() * L.cin / 8;
-            hipLaunchKernelGGL(k_relu_mask, dim3(nblk(n8)), dim3(256), 0, s, n8, L.x, L.gx);
+            NN_LAUNCH(k_relu_mask, dim3(nblk(n8)), dim3(256), 0, s, n8, L.x, L.gx);
         }
     };
     auto up_bwd = [&](int level, int Cu, int Ct, const bf16* dU, const bf16* Lo, bf16* dL) {
         const long long nq = (long long)nb * n->V(level + 1);
-        hipLaunchKernelGGL(k_up_bwd, dim3(nblk(nq * (Cu / 8))), dim3(256), 0, s, (int)nq, n->S[0] >> level,
+        NN_LAUNCH(k_up_bwd, dim3(nblk(nq * (Cu / 8))), dim3(256), 0, s, (int)nq, n->S[0] >> level,
                            n->S[1] >> level, n->S[2] >> level, Cu, Ct, dU, Lo, dL);
     };
     auto pool_skip_bwd = [&](int level, int C, int Cu, int Ct, const bf16* dP, const bf16* dU, const bf16* E,
                              bf16* dE) {
         const long long np = (long long)nb * n->V(level);
-        hipLaunchKernelGGL(k_pool_skip_bwd, dim3(nblk(np * (C / 8))), dim3(256), 0, s, (int)np, n->S[0] >> level,
+        NN_LAUNCH(k_pool_skip_bwd, dim3(nblk(np * (C / 8))), dim3(256), 0, s, (int)np, n->S[0] >> level,
                            n->S[1] >> level, n->S[2] >> level, C, Cu, Ct, dP, dU, E, dE);
     };
     conv_bwd(13, true, true);                        // -> g_d1a
@@ -970,9 +1036,9 @@ void adam_step(Net* n, float lr) {
     ++n->adam_t;
     const float c1 = 1.f / (1.f - std::pow(b1, (float)n->adam_t));
     const float c2 = 1.f / (1.f - std::pow(b2, (float)n->adam_t));
-    hipLaunchKernelGGL(k_grad_sq, dim3(512), dim3(256), 0, n->stream, n->nparam, n->grad, n->clip);
-    hipLaunchKernelGGL(k_clip_factor, dim3(1), dim3(1), 0, n->stream, n->clip);
-    hipLaunchKernelGGL(k_adam, dim3(nblk(n->nparam)), dim3(256), 0, n->stream, n->nparam, n->param, n->grad, n->am,
+    NN_LAUNCH(k_grad_sq, dim3(512), dim3(256), 0, n->stream, n->nparam, n->grad, n->clip);
+    NN_LAUNCH(k_clip_factor, dim3(1), dim3(1), 0, n->stream, n->clip);
+    NN_LAUNCH(k_adam, dim3(nblk(n->nparam)), dim3(256), 0, n->stream, n->nparam, n->param, n->grad, n->am,
                        n->av, lr, b1, b2, c1, c2, eps, (const double*)(n->clip + 2));
     cast_weights(n);
 }
@@ -1015,10 +1081,10 @@ void stat_from_sums(const double s[N_STAT], double V, double st[N_STAT]) {
 // statistics of a binary64 volume and its update on the device
 int device_stats(Net* n, const double* o, const double* cr, double* const u[N_OUT], long long V, double st[N_STAT]) {
     HIPCHK(hipMemsetAsync(n->dacc, 0, 10 * 8, n->stream));
-    hipLaunchKernelGGL(k_moments_d, dim3(512), dim3(256), 0, n->stream, V, o, n->dacc);
-    hipLaunchKernelGGL(k_moments_d, dim3(512), dim3(256), 0, n->stream, V, cr, n->dacc + 2);
+    NN_LAUNCH(k_moments_d, dim3(512), dim3(256), 0, n->stream, V, o, n->dacc);
+    NN_LAUNCH(k_moments_d, dim3(512), dim3(256), 0, n->stream, V, cr, n->dacc + 2);
     for (int k = 0; k < N_OUT; ++k)
-        hipLaunchKernelGGL(k_moments_d, dim3(512), dim3(256), 0, n->stream, V, (const double*)u[k], n->dacc + 4 + 2 * k);
+        NN_LAUNCH(k_moments_d, dim3(512), dim3(256), 0, n->stream, V, (const double*)u[k], n->dacc + 4 + 2 * k);
     double a[10];
     HIPCHK(hipMemcpyAsync(a, n->dacc, 10 * 8, hipMemcpyDeviceToHost, n->stream));
     HIPCHK(hipStreamSynchronize(n->stream));
@@ -1033,10 +1099,10 @@ void load_sample(Net* n, int slot, int si, bool target, SampleMap m) {
     const long long V = n->V(0);
     const float* in = n->ds_in + (long long)si * N_IN * V;
     const VolNorm vn = vol_norm(n->norm, &n->ds_stat[N_STAT * (size_t)si]);
-    hipLaunchKernelGGL(k_input_map, dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, in, m, vn,
+    NN_LAUNCH(k_input_map, dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, in, m, vn,
                        n->x0 + (long long)slot * V * 32);
     if (target)
-        hipLaunchKernelGGL(k_target, dim3(nblk(V)), dim3(256), 0, n->stream, (int)V,
+        NN_LAUNCH(k_target, dim3(nblk(V)), dim3(256), 0, n->stream, (int)V,
                            n->ds_d + (long long)si * N_OUT * V, in + 4 * V, m, vn, n->tgt + 4LL * slot * V);
 }
 SampleMap identity_map(const Net* n) { return SampleMap{n->S[1], n->S[2], 0, 0, 0, 0}; }
@@ -1062,11 +1128,11 @@ int input_from_ctx(Net* n, Ctx* c, int slot, const double st[N_STAT], VolNorm* v
     *vn = vol_norm(n->norm, st);
     bf16* x0 = n->x0 + (long long)slot * V * 32;
     if (c->te == 8)
-        hipLaunchKernelGGL((k_input<double, double, double>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V,
+        NN_LAUNCH((k_input<double, double, double>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V,
                            c->o[cur], c->cr[cur], c->p[cur], (const double*)c->gb[mask_index(c)], n->du[0], n->du[1],
                            n->du[2], *vn, x0);
     else
-        hipLaunchKernelGGL((k_input<double, float, double>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V,
+        NN_LAUNCH((k_input<double, float, double>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V,
                            c->o[cur], c->cr[cur], c->p[cur], (const float*)c->gb[mask_index(c)], n->du[0], n->du[1],
                            n->du[2], *vn, x0);
     return NICR_OK;
@@ -1079,7 +1145,7 @@ using namespace nicr::nn;
 
 extern "C" {
 
-int nicr_unet_create(int s0, int s1, int s2, int max_batch, int train, int max_samples, uint64_t seed, void** net) {
+static int nicr_unet_create_impl(int s0, int s1, int s2, int max_batch, int train, int max_samples, uint64_t seed, void** net) {
     if (!net) return fail(NICR_E_INVALID, "nicr_unet_create: null handle");
     *net = nullptr;
     if (s0 <= 0 || s1 <= 0 || s2 <= 0 || s0 % 8 || s1 % 8 || s2 % 8)
@@ -1141,7 +1207,7 @@ int nicr_unet_get_params(void* h, float* host) {
     return NICR_OK;
 }
 
-int nicr_unet_set_params(void* h, const float* host) {
+static int nicr_unet_set_params_impl(void* h, const float* host) {
     Net* n = static_cast<Net*>(h);
     if (!n || !host) return fail(NICR_E_INVALID, "null argument");
     HIPCHK(hipMemcpyAsync(n->param, host, n->nparam * 4, hipMemcpyHostToDevice, n->stream));
@@ -1173,6 +1239,7 @@ int nicr_unet_set_norm(void* h, const nicr_unet_norm* nm) {
         if (!(nm->c_scale[k] > 0.f)) return fail(NICR_E_INVALID, "nicr_unet_set_norm: scales must be > 0");
     for (int k = 0; k < N_OUT; ++k)
         if (!(nm->gain[k] >= 0.f)) return fail(NICR_E_INVALID, "nicr_unet_set_norm: gains must be >= 0");
+    if (nm->stat_planes < 0) return fail(NICR_E_INVALID, "nicr_unet_set_norm: stat_planes must be >= 0");
     n->norm = *nm;
     return NICR_OK;
 }
@@ -1184,7 +1251,7 @@ int nicr_unet_get_norm(void* h, nicr_unet_norm* nm) {
     return NICR_OK;
 }
 
-int nicr_unet_forward(void* h, int nb, const float* in, float* out) {
+static int nicr_unet_forward_impl(void* h, int nb, const float* in, float* out) {
     Net* n = static_cast<Net*>(h);
     if (!n || !in || !out) return fail(NICR_E_INVALID, "null argument");
     if (nb < 1 || nb > n->nbmax) return fail(NICR_E_INVALID, "nicr_unet_forward: batch exceeds the network's maximum");
@@ -1209,7 +1276,7 @@ int nicr_unet_forward(void* h, int nb, const float* in, float* out) {
         stat_from_sums(s, (double)V, st);
         if (hipMemcpyAsync(tmp, src, (size_t)V * N_IN * 4, hipMemcpyHostToDevice, n->stream) != hipSuccess)
             rc = fail(NICR_E_DEVICE, "nicr_unet_forward: upload failed");
-        hipLaunchKernelGGL((k_input<float, float, float>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, tmp,
+        NN_LAUNCH((k_input<float, float, float>), dim3(nblk(V)), dim3(256), 0, n->stream, (int)V, tmp,
                            tmp + V, tmp + 2 * V, tmp + 3 * V, tmp + 4 * V, tmp + 5 * V, tmp + 6 * V,
                            vol_norm(n->norm, st), n->x0 + (long long)b * V * 32);
         if (hipStreamSynchronize(n->stream) != hipSuccess) rc = fail(NICR_E_DEVICE, "nicr_unet_forward: input");
@@ -1228,7 +1295,7 @@ int nicr_unet_forward(void* h, int nb, const float* in, float* out) {
     return rc;
 }
 
-int nicr_unet_loss_backward(void* h, int nb, const float* in, const float* target, double* loss) {
+static int nicr_unet_loss_backward_impl(void* h, int nb, const float* in, const float* target, double* loss) {
     Net* n = static_cast<Net*>(h);
     if (!n || !in || !target || !loss) return fail(NICR_E_INVALID, "null argument");
     if (!n->train) return fail(NICR_E_STATE, "nicr_unet_loss_backward: network created without training buffers");
@@ -1273,7 +1340,7 @@ int nicr_unet_forward_ms(void* h, double* ms) {
 // gradient dx = conv^T(dy, w) [P][cin]; op 2 weight gradient dw [27][cout][cin]
 // from x and dy. x [P][cin], dy [P][cout], w [27][cout][cin], b [cout]; P = nb
 // s0 s1 s2 positions in the network's cell order.
-int nicr_nn_conv_check(int op, int nb, int s0, int s1, int s2, int cin, int cout, int relu, const float* x,
+static int nicr_nn_conv_check_impl(int op, int nb, int s0, int s1, int s2, int cin, int cout, int relu, const float* x,
                        const float* w, const float* b, const float* dy, float* out) {
     if (op < 0 || op > 2 || nb < 1 || s0 < 1 || s1 < 1 || s2 < 1 || cin % 32 || cout % 32 || cin < 32 || cout < 32 ||
         !w || !out)
@@ -1307,7 +1374,7 @@ int nicr_nn_conv_check(int op, int nb, int s0, int s1, int s2, int cin, int cout
     } else {
         up(wf, w, nw * 4);
         if (!rc)
-            hipLaunchKernelGGL(k_cast_w, dim3(nblk(nw)), dim3(256), 0, n->stream, 27, cout, cin, wf, wb, wt);
+            NN_LAUNCH(k_cast_w, dim3(nblk(nw)), dim3(256), 0, n->stream, 27, cout, cin, wf, wb, wt);
         std::vector<bf16> hx, hdy;
         if (x) {
             hx = to_bf(x, P * cin);
@@ -1352,7 +1419,7 @@ static int sur_buffers(Net* n);
 
 // ---- training set ----
 
-int nicr_sur_record(void* h, void* model, int jump, nicr_step_report* rep) {
+static int nicr_sur_record_impl(void* h, void* model, int jump, nicr_step_report* rep) {
     Net* n = static_cast<Net*>(h);
     Ctx* c = static_cast<Ctx*>(model);
     if (!n || !c || jump < 1) return fail(NICR_E_INVALID, "nicr_sur_record: bad argument");
@@ -1372,20 +1439,20 @@ int nicr_sur_record(void* h, void* model, int jump, nicr_step_report* rep) {
         return NICR_OK;
     }
     const int cur = c->cur;
-    hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], c->o[cur ^ 1], n->du[0]);
-    hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[cur], c->cr[cur ^ 1], n->du[1]);
-    hipLaunchKernelGGL(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->p[cur], c->p[cur ^ 1], n->du[2]);
+    NN_LAUNCH(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], c->o[cur ^ 1], n->du[0]);
+    NN_LAUNCH(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[cur], c->cr[cur ^ 1], n->du[1]);
+    NN_LAUNCH(k_sub, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->p[cur], c->p[cur ^ 1], n->du[2]);
     float* in = n->ds_in + (long long)n->ds_n * N_IN * V;
-    hipLaunchKernelGGL(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], in);
-    hipLaunchKernelGGL(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[cur], in + V);
-    hipLaunchKernelGGL(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->p[cur], in + 2 * V);
+    NN_LAUNCH(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], in);
+    NN_LAUNCH(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[cur], in + V);
+    NN_LAUNCH(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->p[cur], in + 2 * V);
     if (c->te == 8)
-        hipLaunchKernelGGL(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V,
+        NN_LAUNCH(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V,
                            (const double*)c->gb[mask_index(c)], in + 3 * V);
     else
         HIPCHK(hipMemcpyAsync(in + 3 * V, c->gb[mask_index(c)], V * 4, hipMemcpyDeviceToDevice, n->stream));
     for (int k = 0; k < N_OUT; ++k)
-        hipLaunchKernelGGL(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, (const double*)n->du[k],
+        NN_LAUNCH(k_to_float<double>, dim3(nblk(V)), dim3(256), 0, n->stream, V, (const double*)n->du[k],
                            in + (N_FIELD + k) * V);
     HIPCHK(hipMemcpyAsync(n->bk[0], c->o[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
     HIPCHK(hipMemcpyAsync(n->bk[1], c->cr[cur], V * 8, hipMemcpyDeviceToDevice, n->stream));
@@ -1397,9 +1464,9 @@ int nicr_sur_record(void* h, void* model, int jump, nicr_step_report* rep) {
     if (r.status != NICR_STEP_OK) return NICR_OK;   // not recorded
     const int nc = c->cur;
     float* d = n->ds_d + (long long)n->ds_n * N_OUT * V;
-    hipLaunchKernelGGL(k_delta_float, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[nc], n->bk[0], d);
-    hipLaunchKernelGGL(k_delta_float, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[nc], n->bk[1], d + V);
-    hipLaunchKernelGGL(k_delta_float, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->p[nc], n->bk[2], d + 2 * V);
+    NN_LAUNCH(k_delta_float, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[nc], n->bk[0], d);
+    NN_LAUNCH(k_delta_float, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->cr[nc], n->bk[1], d + V);
+    NN_LAUNCH(k_delta_float, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->p[nc], n->bk[2], d + 2 * V);
     HIPCHK(hipStreamSynchronize(n->stream));
     n->ds_stat.resize(N_STAT * (size_t)(n->ds_n + 1));
     for (int k = 0; k < N_STAT; ++k) n->ds_stat[N_STAT * (size_t)n->ds_n + k] = st[k];
@@ -1419,7 +1486,7 @@ int nicr_sur_samples(void* h, int* count) {
 // deviations over all samples, means and standard deviations of the volume
 // statistics over the samples, and output gains that give each normalised
 // correction unit mean square (0 for a field that never changes).
-int nicr_sur_fit_norm(void* h) {
+static int nicr_sur_fit_norm_impl(void* h) {
     Net* n = static_cast<Net*>(h);
     if (!n || n->ds_n < 1) return fail(NICR_E_STATE, "nicr_sur_fit_norm: empty training set");
     const long long V = n->V(0);
@@ -1432,9 +1499,9 @@ int nicr_sur_fit_norm(void* h) {
         HIPCHK(hipMemsetAsync(n->dacc, 0, NM * 8, n->stream));
         const float* in = n->ds_in + (long long)s * N_IN * V;
         for (int c = 0; c < 2; ++c)
-            hipLaunchKernelGGL(k_moments, dim3(256), dim3(256), 0, n->stream, V, in + (2 + c) * V, n->dacc + 2 * c);
+            NN_LAUNCH(k_moments, dim3(256), dim3(256), 0, n->stream, V, in + (2 + c) * V, n->dacc + 2 * c);
         for (int c = 0; c < N_OUT; ++c)
-            hipLaunchKernelGGL(k_corr_sq, dim3(256), dim3(256), 0, n->stream, V,
+            NN_LAUNCH(k_corr_sq, dim3(256), dim3(256), 0, n->stream, V,
                                n->ds_d + ((long long)s * N_OUT + c) * V, in + (N_FIELD + c) * V, J, n->dacc + 4 + c);
         HIPCHK(hipMemcpyAsync(&m[(size_t)s * NM], n->dacc, NM * 8, hipMemcpyDeviceToHost, n->stream));
     }
@@ -1478,6 +1545,7 @@ int nicr_sur_fit_norm(void* h) {
         nm.gain[k] = rms > 1e-300 ? (float)rms : 0.f;
     }
     nm.jump = n->ds_jump;
+    nm.stat_planes = n->S[0];
     return nicr_unet_set_norm(h, &nm);
 }
 
@@ -1491,7 +1559,7 @@ int nicr_sur_set_augment(void* h, int on) {
     return NICR_OK;
 }
 
-int nicr_sur_train(void* h, int iters, int batch, float lr, uint64_t seed, float* losses) {
+static int nicr_sur_train_impl(void* h, int iters, int batch, float lr, uint64_t seed, float* losses) {
     Net* n = static_cast<Net*>(h);
     if (!n || iters < 0) return fail(NICR_E_INVALID, "nicr_sur_train: bad argument");
     if (!n->train) return fail(NICR_E_STATE, "nicr_sur_train: network created without training buffers");
@@ -1525,6 +1593,7 @@ int nicr_sur_train(void* h, int iters, int batch, float lr, uint64_t seed, float
         HIPCHK(hipMemsetAsync(n->grad, 0, n->nparam * 4, n->stream));
         backward(n, batch, n->loss_acc + it);
         adam_step(n, lr);
+        if (g_launch.failed) break;   // reported by the entry point
     }
     HIPCHK(hipGetLastError());
     if (losses) {
@@ -1539,7 +1608,7 @@ int nicr_sur_train(void* h, int iters, int batch, float lr, uint64_t seed, float
 
 // Mean squared error of the normalised deltas over training samples [first,
 // first + count) (no parameter update).
-int nicr_sur_eval(void* h, int first, int count, double* mse) {
+static int nicr_sur_eval_impl(void* h, int first, int count, double* mse) {
     Net* n = static_cast<Net*>(h);
     if (!n || !mse || first < 0 || count < 1 || first + count > n->ds_n)
         return fail(NICR_E_INVALID, "nicr_sur_eval: bad sample range");
@@ -1643,7 +1712,7 @@ struct SpareEta {
     }
 };
 
-int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* opt, nicr_sur_report* rep) {
+static int nicr_sur_advance_impl(void* h, void* model, int nsteps, const nicr_sur_options* opt, nicr_sur_report* rep) {
     Net* n = static_cast<Net*>(h);
     Ctx* c = static_cast<Ctx*>(model);
     if (!n || !c || !opt || !rep || nsteps < 0) return fail(NICR_E_INVALID, "nicr_sur_advance: bad argument");
@@ -1714,16 +1783,28 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
     };
     if (opt->settle_steps < 0) return fail(NICR_E_INVALID, "nicr_sur_advance: settle_steps must be >= 0");
     if (opt->fallback_steps < 0) return fail(NICR_E_INVALID, "nicr_sur_advance: fallback_steps must be >= 0");
+    if (opt->min_jump < 0) return fail(NICR_E_INVALID, "nicr_sur_advance: min_jump must be >= 0");
+    // shortest last jump of a call (none unless 2 <= min_jump < jump)
+    const int min_jump = opt->min_jump >= 2 && opt->min_jump < J ? opt->min_jump : J;
     const int fallback = opt->fallback_steps > 0 ? opt->fallback_steps : J;
+    // NICR_SUR_TRACE=1: one line per jump on stderr (steps left, jump length,
+    // residuals of O, Cr, eta, the decision)
+    const char* tr = std::getenv("NICR_SUR_TRACE");
+    const bool trace = tr && tr[0] == '1';
     // the update u0 of a jump is the difference of the two state buffers, so
     // they must hold consecutive states: true after the check step of an
     // accepted jump and after plain steps, not on entry
     bool consecutive = false;
     while (left > 0 && rep->status == NICR_STEP_OK) {
         const int pre = consecutive ? opt->settle_steps : 1;
+        // this cycle's jump: a full one, or a last shorter one of the steps left
+        int Jc = J;
         if (left < J + 1 + pre) {
-            if ((rc = plain(left))) return rc;
-            break;
+            Jc = left - 1 - pre;
+            if (Jc < min_jump) {
+                if ((rc = plain(left))) return rc;
+                break;
+            }
         }
         // plain steps before the jump (none when the last step already gives u0)
         if ((rc = plain_tracked(pre))) return rc;
@@ -1732,7 +1813,7 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         if ((rc = sync_ctl(c, &saved))) return rc;   // also: the step has finished
         const int cur = c->cur;
         // one pass: u0, backup of x0, volume statistics and O/Cr totals
-        hipLaunchKernelGGL(k_sur_pre, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, V, c->o[cur], c->cr[cur], c->p[cur],
+        NN_LAUNCH(k_sur_pre, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, V, c->o[cur], c->cr[cur], c->p[cur],
                            c->o[cur ^ 1], c->cr[cur ^ 1], c->p[cur ^ 1], n->du[0], n->du[1], n->du[2], n->bk[0],
                            n->bk[1], n->bk[2], n->part);
         long double ps[7];
@@ -1741,12 +1822,26 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         const double sums[N_STAT] = {(double)ps[0], (double)ps[1], (double)ps[2], (double)ps[3],
                                      (double)ps[4], (double)ps[5], (double)ps[6]};
         double st[N_STAT];
-        stat_from_sums(sums, (double)V, st);
+        // the network's statistics: over the whole volume, or over its first
+        // stat_planes X planes (cells in X-major order, so a prefix)
+        const long long nstat = n->norm.stat_planes > 0 && n->norm.stat_planes < n->S[0]
+                                    ? (long long)n->norm.stat_planes * n->S[1] * n->S[2] : V;
+        if (nstat < V) {
+            if ((rc = device_stats(n, c->o[cur], c->cr[cur], n->du, nstat, st))) return rc;
+        } else {
+            stat_from_sums(sums, (double)V, st);
+        }
         // proposal x0 + J u0 + network correction, in place in the current buffers
         VolNorm vn;
         if ((rc = input_from_ctx(n, c, 0, st, &vn))) return rc;
+        if (Jc != J) {
+            // a shorter jump: Jc u0 and the correction scaled by (Jc / J)^2
+            const double r = (double)Jc / (double)J;
+            vn.jump = (double)Jc;
+            for (int k = 0; k < N_OUT; ++k) vn.out_scale[k] *= r * r;
+        }
         forward(n, 1);
-        hipLaunchKernelGGL(k_sur_apply, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, V, n->out, vn, n->du[0], n->du[1],
+        NN_LAUNCH(k_sur_apply, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, V, n->out, vn, n->du[0], n->du[1],
                            n->du[2], c->o[cur], c->cr[cur], c->p[cur], n->part);
         long double pa[8];
         if ((rc = sur_partials(n, 8, pa, 2))) return rc;
@@ -1763,12 +1858,12 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         // at x0); the check step's sources refine this below
         double f[2] = {1., 1.};
         const double s1[2] = {(double)pa[0], (double)pa[1]};
-        const double target[2] = {(double)((long double)m0[0] + (long double)J * species_sum(src_last, 0)),
-                                  (double)((long double)m0[1] + (long double)J * species_sum(src_last, 1))};
+        const double target[2] = {(double)((long double)m0[0] + (long double)Jc * species_sum(src_last, 0)),
+                                  (double)((long double)m0[1] + (long double)Jc * species_sum(src_last, 1))};
         if (conserve) {
             for (int k = 0; k < 2; ++k)
                 if (s1[k] > 0. && target[k] > 0.) f[k] = target[k] / s1[k];
-            hipLaunchKernelGGL(k_scale2, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], f[0], c->cr[cur], f[1]);
+            NN_LAUNCH(k_scale2, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[cur], f[0], c->cr[cur], f[1]);
             HIPCHK(hipStreamSynchronize(n->stream));
         }
         const double omin = (double)pa[2] * f[0], omax = -(double)pa[3] * f[0];
@@ -1784,8 +1879,8 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
             // the current mask buffers (the old one backed up); eta_-1 (the
             // other buffer) is overwritten by the check step
             const Par& P = c->P;
-            hipLaunchKernelGGL(k_sur_eta_pre, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, c->ncell, P.fstride,
-                               c->cfg.n_eta, (double)J, P.eta_scale, (const double*)c->eta[cur ^ 1],
+            NN_LAUNCH(k_sur_eta_pre, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, c->ncell, P.fstride,
+                               c->cfg.n_eta, (double)Jc, P.eta_scale, (const double*)c->eta[cur ^ 1],
                                (const double*)c->eta[cur], (double*)spare, (double*)c->s2[cur],
                                (double*)c->pairs[cur], (double*)c->gb[cur], n->bk_mask, n->part);
             std::swap(c->eta[cur], spare);
@@ -1801,15 +1896,15 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         }
         if (ok) {
             const int k = c->cur;   // proposal in k ^ 1, one step after it in k
-            hipLaunchKernelGGL(k_sur_trap, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, V, c->o[k ^ 1], c->o[k],
-                               n->bk[0], n->du[0], c->cr[k ^ 1], c->cr[k], n->bk[1], n->du[1], 0.5 * J, n->part);
+            NN_LAUNCH(k_sur_trap, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, V, c->o[k ^ 1], c->o[k],
+                               n->bk[0], n->du[0], c->cr[k ^ 1], c->cr[k], n->bk[1], n->du[1], 0.5 * Jc, n->part);
             long double pt[4];
             if ((rc = sur_partials(n, 4, pt))) return rc;
             ro = pt[1] > 0 ? (double)std::sqrt(pt[0] / pt[1]) : INFINITY;
             rcr = pt[3] > 0 ? (double)std::sqrt(pt[2] / pt[3]) : INFINITY;
             if (evo) {
-                hipLaunchKernelGGL(k_sur_eta_trap, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, c->ncell,
-                                   c->P.fstride, c->cfg.n_eta, (double)J, (const double*)c->eta[k ^ 1],
+                NN_LAUNCH(k_sur_eta_trap, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, c->ncell,
+                                   c->P.fstride, c->cfg.n_eta, (double)Jc, (const double*)c->eta[k ^ 1],
                                    (const double*)c->eta[k], (const double*)spare, n->part);
                 long double pr[2];
                 if ((rc = sur_partials(n, 2, pr))) return rc;
@@ -1820,6 +1915,9 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
         rep->last_residual[0] = ro;
         rep->last_residual[1] = rcr;
         rep->last_residual[2] = reta;
+        if (trace)
+            std::fprintf(stderr, "[sur] left %d jump %d pre %d residual O %.4g Cr %.4g eta %.4g %s\n", left, Jc, pre,
+                         ro, rcr, reta, ok ? "accepted" : "rejected");
         if (ok) {
             // the jump's sources by the trapezoid rule over its `jump` steps,
             // into the budgets; with conserve_mass the state after the check
@@ -1830,7 +1928,7 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
             long double src_check[NSLOT + 1];
             sources(saved, after, src_check);
             long double add[NSLOT + 1];
-            for (int k = 0; k <= NSLOT; ++k) add[k] = 0.5L * (long double)J * (src_last[k] + src_check[k]);
+            for (int k = 0; k <= NSLOT; ++k) add[k] = 0.5L * (long double)Jc * (src_last[k] + src_check[k]);
             bool changed = false;
             for (int k = 0; k < NSLOT; ++k) {
                 if (add[k] == 0.L) continue;
@@ -1857,7 +1955,7 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
                 }
                 if (g[0] != 1. || g[1] != 1.) {
                     const int k = c->cur;
-                    hipLaunchKernelGGL(k_scale2, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[k], g[0], c->cr[k],
+                    NN_LAUNCH(k_scale2, dim3(nblk(V)), dim3(256), 0, n->stream, V, c->o[k], g[0], c->cr[k],
                                        g[1]);
                     HIPCHK(hipStreamSynchronize(n->stream));
                 }
@@ -1868,7 +1966,7 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
             rep->max_residual_cr = std::max(rep->max_residual_cr, rcr);
             rep->max_residual_eta = std::max(rep->max_residual_eta, reta);
             rep->plain_steps += 1;
-            left -= J + 1;
+            left -= Jc + 1;
             consecutive = true;
             continue;
         }
@@ -1901,6 +1999,70 @@ int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* o
     }
     rep->steps_done = nsteps - left;
     return NICR_OK;
+}
+
+// Entry points that launch kernels: the launch checks of the call decide its
+// result when the call itself succeeded.
+int nicr_unet_create(int s0, int s1, int s2, int max_batch, int train, int max_samples, uint64_t seed, void** net) {
+    nicr::nn::g_launch = {};
+    const int rc = nicr_unet_create_impl(s0, s1, s2, max_batch, train, max_samples, seed, net);
+    const int lc = nicr::nn::launch_status();
+    return rc ? rc : lc;
+}
+int nicr_unet_set_params(void* h, const float* host) {
+    nicr::nn::g_launch = {};
+    const int rc = nicr_unet_set_params_impl(h, host);
+    const int lc = nicr::nn::launch_status();
+    return rc ? rc : lc;
+}
+int nicr_unet_forward(void* h, int nb, const float* in, float* out) {
+    nicr::nn::g_launch = {};
+    const int rc = nicr_unet_forward_impl(h, nb, in, out);
+    const int lc = nicr::nn::launch_status();
+    return rc ? rc : lc;
+}
+int nicr_unet_loss_backward(void* h, int nb, const float* in, const float* target, double* loss) {
+    nicr::nn::g_launch = {};
+    const int rc = nicr_unet_loss_backward_impl(h, nb, in, target, loss);
+    const int lc = nicr::nn::launch_status();
+    return rc ? rc : lc;
+}
+int nicr_nn_conv_check(int op, int nb, int s0, int s1, int s2, int cin, int cout, int relu, const float* x,
+                       const float* w, const float* b, const float* dy, float* out) {
+    nicr::nn::g_launch = {};
+    const int rc = nicr_nn_conv_check_impl(op, nb, s0, s1, s2, cin, cout, relu, x, w, b, dy, out);
+    const int lc = nicr::nn::launch_status();
+    return rc ? rc : lc;
+}
+int nicr_sur_record(void* h, void* model, int jump, nicr_step_report* rep) {
+    nicr::nn::g_launch = {};
+    const int rc = nicr_sur_record_impl(h, model, jump, rep);
+    const int lc = nicr::nn::launch_status();
+    return rc ? rc : lc;
+}
+int nicr_sur_fit_norm(void* h) {
+    nicr::nn::g_launch = {};
+    const int rc = nicr_sur_fit_norm_impl(h);
+    const int lc = nicr::nn::launch_status();
+    return rc ? rc : lc;
+}
+int nicr_sur_train(void* h, int iters, int batch, float lr, uint64_t seed, float* losses) {
+    nicr::nn::g_launch = {};
+    const int rc = nicr_sur_train_impl(h, iters, batch, lr, seed, losses);
+    const int lc = nicr::nn::launch_status();
+    return rc ? rc : lc;
+}
+int nicr_sur_eval(void* h, int first, int count, double* mse) {
+    nicr::nn::g_launch = {};
+    const int rc = nicr_sur_eval_impl(h, first, count, mse);
+    const int lc = nicr::nn::launch_status();
+    return rc ? rc : lc;
+}
+int nicr_sur_advance(void* h, void* model, int nsteps, const nicr_sur_options* opt, nicr_sur_report* rep) {
+    nicr::nn::g_launch = {};
+    const int rc = nicr_sur_advance_impl(h, model, nsteps, opt, rep);
+    const int lc = nicr::nn::launch_status();
+    return rc ? rc : lc;
 }
 
 }  // extern "C"

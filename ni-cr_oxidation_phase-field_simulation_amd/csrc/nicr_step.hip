@@ -975,12 +975,16 @@ eta_value(const Par& P, const EtaCell<PREC, TE>& e, const TE* L, int oym, int oy
         const double dF = ec * (P.etaW * (e.S - (1. - h64(e.pp)) + 2. * (e.S - ec * ec)) + P.etaA * h64(e.pp));
         return ec - e.dtl * (dF - P.etaK * lp);
     } else {
-#pragma clang fp contract(fast)
+        // binary32 with every rounding spelled out (no contraction left to the
+        // compiler), so that every kernel that evaluates a cell gives the same
+        // bits: the row and the 64-wide marching kernels are bitwise equal
         const float ec = (float)cc;
-        const float lp = ((float)nv - 2.f * ec + (float)pv) * P.fdx2i + ((float)L[oyp] - 2.f * ec + (float)L[oym]) * P.fdy2i
-                       + ((float)L[ozp] - 2.f * ec + (float)L[ozm]) * P.fdz2i;
-        const float dF = ec * (e.coef - e.W2 * ec * ec);
-        return ec - e.fdtl * (dF - e.K * lp);
+        const float ax = __fadd_rn(__fmaf_rn(-2.f, ec, (float)nv), (float)pv);
+        const float ay = __fadd_rn(__fmaf_rn(-2.f, ec, (float)L[oyp]), (float)L[oym]);
+        const float az = __fadd_rn(__fmaf_rn(-2.f, ec, (float)L[ozp]), (float)L[ozm]);
+        const float lp = __fmaf_rn(az, P.fdz2i, __fmaf_rn(ay, P.fdy2i, __fmul_rn(ax, P.fdx2i)));
+        const float dF = __fmul_rn(ec, __fmaf_rn(-ec, __fmul_rn(e.W2, ec), e.coef));
+        return __fmaf_rn(-e.fdtl, __fmaf_rn(-lp, e.K, dF), ec);
     }
 }
 
@@ -1025,10 +1029,9 @@ __device__ __forceinline__ void eta_cell_update(const Par& P, const Bufs<TE, TM>
             pr += S2 * q;
             S2 += q;
         } else {
-#pragma clang fp contract(fast)
-            const float q = v * v;
-            pr += S2 * q;
-            S2 += q;
+            const float q = __fmul_rn(v, v);
+            pr = __fmaf_rn(S2, q, pr);
+            S2 = __fadd_rn(S2, q);
         }
         // keep the scheduler from hoisting the LDS reads of all fields at
         // once (register pressure): groups of four fields

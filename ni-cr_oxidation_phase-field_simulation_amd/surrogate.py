@@ -24,18 +24,19 @@ class UNetNorm(ctypes.Structure):
     """nicr_unet_norm: fixed scales of phi, gb, the volume-statistics channels, and output gains."""
     _fields_ = [("phi_mean", ctypes.c_float), ("phi_scale", ctypes.c_float), ("gb_mean", ctypes.c_float),
                 ("gb_scale", ctypes.c_float), ("c_mean", ctypes.c_float * 4), ("c_scale", ctypes.c_float * 4),
-                ("gain", ctypes.c_float * 3), ("jump", ctypes.c_int32)]
+                ("gain", ctypes.c_float * 3), ("jump", ctypes.c_int32), ("stat_planes", ctypes.c_int32)]
 
     def as_dict(self):
         return {"phi_mean": self.phi_mean, "phi_scale": self.phi_scale, "gb_mean": self.gb_mean,
                 "gb_scale": self.gb_scale, "c_mean": list(self.c_mean), "c_scale": list(self.c_scale),
-                "gain": list(self.gain), "jump": self.jump}
+                "gain": list(self.gain), "jump": self.jump, "stat_planes": self.stat_planes}
 
 
 def make_norm(phi_mean=0.0, phi_scale=1.0, gb_mean=0.0, gb_scale=1.0, c_mean=(0.0,) * 4, c_scale=(1.0,) * 4,
-              gain=(1.0,) * 3, jump=50) -> UNetNorm:
+              gain=(1.0,) * 3, jump=50, stat_planes=0) -> UNetNorm:
     nm = UNetNorm()
     nm.jump = int(jump)
+    nm.stat_planes = int(stat_planes)
     nm.phi_mean, nm.phi_scale, nm.gb_mean, nm.gb_scale = phi_mean, phi_scale, gb_mean, gb_scale
     for k in range(4):
         nm.c_mean[k], nm.c_scale[k] = c_mean[k], c_scale[k]
@@ -47,7 +48,7 @@ def make_norm(phi_mean=0.0, phi_scale=1.0, gb_mean=0.0, gb_scale=1.0, c_mean=(0.
 class SurOptions(ctypes.Structure):
     _fields_ = [("jump", ctypes.c_int32), ("conserve_mass", ctypes.c_int32), ("residual_tol", ctypes.c_double),
                 ("min_value", ctypes.c_double), ("max_value", ctypes.c_double), ("settle_steps", ctypes.c_int32),
-                ("fallback_steps", ctypes.c_int32)]
+                ("fallback_steps", ctypes.c_int32), ("min_jump", ctypes.c_int32)]
 
 
 class SurReport(ctypes.Structure):
@@ -232,10 +233,10 @@ class UNet:
 
     # ---- surrogate stepping ----
     def advance(self, model, nsteps: int, jump: int = 50, conserve_mass: bool = True, residual_tol: float = 0.1,
-                bounds=(0.0, 1.0), settle_steps: int = 0, fallback_steps: int = 0) -> SurReport:
+                bounds=(0.0, 1.0), settle_steps: int = 0, fallback_steps: int = 0, min_jump: int = 0) -> SurReport:
         """Advance a binary64 Model by nsteps with accepted network jumps (nicr_sur_advance)."""
         opt = SurOptions(int(jump), int(bool(conserve_mass)), float(residual_tol), float(bounds[0]),
-                         float(bounds[1]), int(settle_steps), int(fallback_steps))
+                         float(bounds[1]), int(settle_steps), int(fallback_steps), int(min_jump))
         rep = SurReport()
         _check(self.lib, self.lib.nicr_sur_advance(self.h, model._h.h, int(nsteps), ctypes.byref(opt),
                                                    ctypes.byref(rep)))
@@ -330,7 +331,7 @@ def closed_model(nicr, shape, state, precision="fp64"):
 
 
 def record_inlet_trajectories(net, nicr, n_traj, jump=50, n_struct=16, diameter=63.0, warm=(20, 5000), seed=0,
-                              log=None, noise=0.0, noise_frac=0.75):
+                              log=None, noise=0.0, noise_frac=0.75, noise_fields=("cr",)):
     """One training sample from each of n_traj plain binary64 trajectories of
     the static-grain inlet workload (the generator of the 128^3 static-grain
     configuration): the reference's Voronoi grains of --grain-diameter-grid
@@ -339,7 +340,9 @@ def record_inlet_trajectories(net, nicr, n_traj, jump=50, n_struct=16, diameter=
     reference schedule. Trajectory t uses grain structure t % n_struct and its
     own composition noise, runs a random number of steps in `warm` of the
     reference loop, then records (state, plain update, change over `jump`
-    steps). noise: as record_trajectories."""
+    steps). noise: as record_trajectories, for the fields in noise_fields only
+    (Cr by default: O is zero ahead of the inlet front, where cell-wise noise
+    is clipped into a one-sided source)."""
     from . import workloads as wl
     from . import synthetic as syn
     shape = net.shape
@@ -357,10 +360,10 @@ def record_inlet_trajectories(net, nicr, n_traj, jump=50, n_struct=16, diameter=
             raise RuntimeError(f"trajectory {t}: a plain step was rejected")
         if noise > 0.0 and r.uniform() < noise_frac:
             c = r.uniform(0.0, noise)
-            before = {k: m.get(k) for k in ("o", "cr")}
+            before = {k: m.get(k) for k in noise_fields}
             m.step(1)
             steps += 1
-            for k in ("o", "cr"):
+            for k in noise_fields:
                 a = m.get(k)
                 amp = c * jump * float(np.sqrt(np.mean((a - before[k]) ** 2)))
                 m.set(k, np.clip(a + amp * r.standard_normal(shape), 0.0, 1.0))
