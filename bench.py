@@ -387,7 +387,8 @@ def main():
     ap.add_argument("--sur-fallback", type=int, default=50, help="plain steps after a rejected jump")
     ap.add_argument("--sur-noise", type=float, default=0.2,
                     help="largest perturbation of the recorded training states' Cr (x jump x RMS update)")
-    ap.add_argument("--sur-tol", type=float, default=0.1, help="trapezoid residual tolerance of a jump")
+    ap.add_argument("--sur-tol", type=float, default=0.15,
+                    help="trapezoid residual tolerance of a jump (final fields stay within 1e-3 relative L2)")
     ap.add_argument("--sur-min-jump", type=int, default=12,
                     help="shortest last jump before a Ksp scan (0: the steps left there are plain steps)")
     ap.add_argument("--sur-net-fields", default="cr,phi",

@@ -308,8 +308,13 @@ __device__ __forceinline__ void block_sums8(double (&v)[NV], double* __restrict_
 }
 
 // Before a jump: u = x0 - x_-1 for O, Cr, phi; backup of x0; sums of O, O^2,
-// Cr, Cr^2, u_O^2, u_Cr^2, u_phi^2.
-__global__ __launch_bounds__(256) void k_sur_pre(long long n, const double* __restrict__ o,
+// Cr, Cr^2, u_O^2, u_Cr^2, u_phi^2. The first n_hold cells (the inlet plane x
+// = 0 under the concentration boundary, whose O every step resets to the
+// inlet value before its update, 2D:408-411) keep u_O = 0: their O after a
+// step is the inlet value plus one step's change, so a jump leaves it as it
+// is; extrapolating it would carry the reset of a check step (when the state
+// before it came from a jump) into the next jump.
+__global__ __launch_bounds__(256) void k_sur_pre(long long n, long long n_hold, const double* __restrict__ o,
                                                  const double* __restrict__ cr, const double* __restrict__ ph,
                                                  const double* __restrict__ op, const double* __restrict__ crp,
                                                  const double* __restrict__ php, double* __restrict__ u0,
@@ -319,7 +324,7 @@ __global__ __launch_bounds__(256) void k_sur_pre(long long n, const double* __re
     double v[7] = {0., 0., 0., 0., 0., 0., 0.};
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         const double a = o[i], c = cr[i], p = ph[i];
-        const double d0 = a - op[i], d1 = c - crp[i], d2 = p - php[i];
+        const double d0 = i < n_hold ? 0. : a - op[i], d1 = c - crp[i], d2 = p - php[i];
         u0[i] = d0;
         u1[i] = d1;
         u2[i] = d2;
@@ -1813,7 +1818,8 @@ static int nicr_sur_advance_impl(void* h, void* model, int nsteps, const nicr_su
         if ((rc = sync_ctl(c, &saved))) return rc;   // also: the step has finished
         const int cur = c->cur;
         // one pass: u0, backup of x0, volume statistics and O/Cr totals
-        NN_LAUNCH(k_sur_pre, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, V, c->o[cur], c->cr[cur], c->p[cur],
+        const long long n_hold = c->P.enforce_layer ? (long long)n->S[1] * n->S[2] : 0;
+        NN_LAUNCH(k_sur_pre, dim3(SUR_BLOCKS), dim3(256), 0, n->stream, V, n_hold, c->o[cur], c->cr[cur], c->p[cur],
                            c->o[cur ^ 1], c->cr[cur ^ 1], c->p[cur ^ 1], n->du[0], n->du[1], n->du[2], n->bk[0],
                            n->bk[1], n->bk[2], n->part);
         long double ps[7];
