@@ -259,6 +259,55 @@ def test_jump_sources_are_trapezoid_of_step_sources(nicr):
     net.close()
 
 
+@pytest.mark.parametrize("jc,min_jump", [(12, 5), (12, 13)], ids=["short-jump", "below-min-jump"])
+def test_last_short_jump(nicr, jc, min_jump):
+    """A call whose steps do not fit a full jump (J = 20): with min_jump <= Jc
+    the last cycle jumps Jc = steps - 2 (one plain step for u0, the check
+    step): bitwise the extrapolation x0 + Jc (x0 - x_-1) (O held on the inlet
+    plane) uploaded to a twin and stepped once, budgets the twin's plus Jc/2
+    (s_pre + s_check); with min_jump > Jc every step is a plain step."""
+    J = 20
+    shape = (16, 16, 24)
+    a = _open_model(nicr, shape, 7)
+    b = _open_model(nicr, shape, 7)
+    a.step(150)
+    b.step(150)
+    names = ("o", "cr", "p")
+    net = _sur().UNet(shape, seed=1)
+    _plain_norm(net, J)
+    rep = net.advance(a, jc + 2, jump=J, conserve_mass=False, residual_tol=1e30, bounds=(-1.0, 2.0),
+                      min_jump=min_jump)
+    assert rep.status == 0 and rep.steps_done == jc + 2
+    if min_jump > jc:
+        assert rep.jumps_accepted == 0 and rep.plain_steps == jc + 2
+        b.step(jc + 2)
+        for f in names:
+            assert np.array_equal(a.get(f), b.get(f)), f
+        assert a.b == b.b
+        net.close()
+        return
+    assert rep.jumps_accepted == 1 and rep.plain_steps == 2
+    xm = {f: b.get(f) for f in names}
+    bm = b.b
+    b.step(1)
+    x0 = {f: b.get(f) for f in names}
+    b0 = b.b
+    for f in names:
+        d = x0[f] - xm[f]
+        if f == "o":
+            d[0] = 0.0
+        b.set(f, x0[f] + jc * d)
+    b.step(1)
+    b1 = b.b
+    for f in names:
+        assert np.array_equal(a.get(f), b.get(f)), f
+    ba = a.b
+    for k in ("bcO", "ckO", "clipO", "bcCr", "ckCr", "clipCr"):
+        want = b1[k] + 0.5 * jc * ((b0[k] - bm[k]) + (b1[k] - b0[k]))
+        assert ba[k] == pytest.approx(want, rel=1e-14, abs=1e-300), k
+    net.close()
+
+
 @pytest.mark.parametrize("change", [dict(), dict(bc="mu"), dict(bc="closed")],
                          ids=["inlet-ck-ksp", "mu-inlet", "closed-ck-ksp"])
 def test_open_model_jumps_carry_sources(nicr, change):
