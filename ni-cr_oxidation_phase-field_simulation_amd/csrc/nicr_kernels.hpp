@@ -180,6 +180,10 @@ __device__ __forceinline__ DD dd_add(DD a, DD b) {
 // address is base + soffset (scalar) + voffset (lane), so one lane offset can
 // serve many fields whose distance is a uniform scalar.
 constexpr int BUF_DWORD3 = 0x00020000;
+// s_waitcnt immediate (gfx9 encoding) for vmcnt(0) with the other counters
+// not waited on: expcnt 7 (bits 6:4), lgkmcnt 15 (bits 11:8), vmcnt 0 (bits
+// 3:0 and 15:14)
+constexpr int WAIT_VM0 = 0x0F70;
 template <class T>
 __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
 template <>

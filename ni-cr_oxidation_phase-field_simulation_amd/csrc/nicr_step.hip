@@ -380,10 +380,12 @@ __device__ __forceinline__ double h64_fma(double p) {
     return p * p * p * (10. + p * (-15. + 6. * p));
 }
 
+// finish_cell_vals leaves the final values in onv, crv, pnv for the caller to
+// store; finish_cell stores them.
 template <int PREC = PREC_FP64>
-__device__ __forceinline__ void finish_cell(const Par& P, long long c, int x, double o_stored, double oo, double cc,
-                                            double pp, double onv, double crv, double pnv, double left_flux,
-                                            StepAcc& a, double* on, double* crn, double* pn, double layer_o) {
+__device__ __forceinline__ void finish_cell_vals(const Par& P, long long c, int x, double o_stored, double oo,
+                                                 double cc, double pp, double& onv, double& crv, double& pnv,
+                                                 double left_flux, StepAcc& a, double layer_o) {
     if (P.ck_legacy) {
         double dO, dCr;
         if constexpr (PREC == PREC_FP64) {
@@ -425,6 +427,12 @@ __device__ __forceinline__ void finish_cell(const Par& P, long long c, int x, do
     } else if (onv < -1.e-12 || crv < -1.e-12 || onv + crv > 1. + 1.e-12 || pnv < -1.e-12 || pnv > 1. + 1.e-12) {
         a.bad = a.bad < (unsigned long long)c ? a.bad : (unsigned long long)c;
     }
+}
+template <int PREC = PREC_FP64>
+__device__ __forceinline__ void finish_cell(const Par& P, long long c, int x, double o_stored, double oo, double cc,
+                                            double pp, double onv, double crv, double pnv, double left_flux,
+                                            StepAcc& a, double* on, double* crn, double* pn, double layer_o) {
+    finish_cell_vals<PREC>(P, c, x, o_stored, oo, cc, pp, onv, crv, pnv, left_flux, a, layer_o);
     st_nt(&on[c], onv);
     st_nt(&crn[c], crv);
     st_nt(&pn[c], pnv);
@@ -751,22 +759,32 @@ __global__ __launch_bounds__(64 * TY) void k_pde_march(Par P, Bufs<TE, TM> B, in
         p7[SZP] = sp[sp_x * P1 + own1 + 1];
         p7[SXP] = sp[sp_x1 * P1 + own1];
         p7[SXM] = p_prev;
-        if constexpr (PREC != PREC_FP64) {
-            if (active && B.ecoef) B.ecoef[c] = eta_coef32(P, (float)S, (float)p7[SC]);
-        }
+        // the cell's results are stored after the staging below: the staging
+        // waits for this plane's prefetch loads, and stores issued before it
+        // would be waited for as well
+        float ecoef_v = 0.f;
+        if constexpr (PREC != PREC_FP64) ecoef_v = eta_coef32(P, (float)S, (float)p7[SC]);
+        double onv = 0., crv = 0., pnv = 0.;
         if (active) {
             const double g_c = x == 0 ? (double)gb[c] : 0.;
             const double o_stored = x == 0 ? o[c] : oo;
-            double onv, crv, pnv, left_flux;
+            double left_flux;
             trial_mm<3, PREC, T>(P, m, p7, oo, cc, x, S, g_c, m64O, onv, crv, pnv, left_flux);
-            finish_cell<PREC>(P, c, x, o_stored, oo, cc, p7[SC], onv, crv, pnv, left_flux, acc, B.o[nxt], B.cr[nxt],
-                        B.p[nxt], P.layer_o);
+            finish_cell_vals<PREC>(P, c, x, o_stored, oo, cc, p7[SC], onv, crv, pnv, left_flux, acc, P.layer_o);
         }
         m_prev = m[SC];
         p_prev = p7[SC];
         // D: staged planes into the slots of planes x (o, cr; gb) and x-1 (phi)
         store_oc(so_x);
         store_pg(sp_x2, m0);
+        if (active) {
+            st_nt(&B.o[nxt][c], onv);
+            st_nt(&B.cr[nxt][c], crv);
+            st_nt(&B.p[nxt][c], pnv);
+            if constexpr (PREC != PREC_FP64) {
+                if (B.ecoef) B.ecoef[c] = ecoef_v;
+            }
+        }
         __syncthreads();
         const int t = so_x;
         so_x = so_x1;
@@ -1041,8 +1059,7 @@ __device__ __forceinline__ void eta_cell_update(const Par& P, const Bufs<TE, TM>
     // sinks their chain below it, which keeps all NG trial values live across
     // the slow path and pushes the kernel past its register budget.
     if constexpr (PREC != PREC_FP64) asm volatile("" : "+v"(S2), "+v"(pr));
-    if (!active) return;
-    if (PREC != PREC_FP64 && !ok) {
+    if (PREC != PREC_FP64 && active && !ok) {
         for (int g = 0; g < NG; ++g) {
             const V v = eta_value<PREC>(P, e, lds + g * PL, oym, oyp, ozm, ozp, prev[g], cv[g], nxv[g]);
             if (!(v >= lo && v <= hi)) {
@@ -1052,12 +1069,19 @@ __device__ __forceinline__ void eta_cell_update(const Par& P, const Bufs<TE, TM>
             }
         }
     }
-    st_nt_off(B.s2[nxt_i], cbo, (TE)S2);
-    if (!last_pass) st_nt_off(B.pairs[nxt_i], cbo, (TE)pr);   // the next pass's running sum
-    if (last_pass) {
-        const V s16 = (V)16 * pr;
-        st_nt_off(B.gb[nxt_i], cbo, (TE)((V)P.eta_scale * (s16 < (V)1 ? s16 : (V)1)));
-    }
+    // The mask sums: S, then the next pass's running sum P or, after the last
+    // pass, gb. Stored through buffer resources, lanes past the mesh edge out
+    // of range as above, so that no branch separates these stores from the
+    // next plane's loads (a branch there makes the compiler's wait for those
+    // loads wait for the stores as well).
+    const int nb = (int)((unsigned)P.ncell * (unsigned)sizeof(TE));
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(B.s2[nxt_i], 0, nb, BUF_DWORD3);
+    const __amdgpu_buffer_rsrc_t rq =
+        __builtin_amdgcn_make_buffer_rsrc(last_pass ? B.gb[nxt_i] : B.pairs[nxt_i], 0, nb, BUF_DWORD3);
+    const V s16 = (V)16 * pr;
+    const V q = last_pass ? (V)P.eta_scale * (s16 < (V)1 ? s16 : (V)1) : pr;
+    buf_store<TE>(rs, cb, 0u, (TE)S2);
+    buf_store<TE>(rq, cb, 0u, (TE)q);
 }
 
 // ---------------------------------------------------------------------------
@@ -1283,6 +1307,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(eta_rows_wa
         }
     };
     if (AHEAD) load_halo(x0);
+    // The prologue's loads are needed by the first plane anyway: waiting for
+    // them here keeps them out of the compiler's count at the loop head, which
+    // would otherwise make every plane wait for its own stores there.
+    __builtin_amdgcn_s_waitcnt(WAIT_VM0);
     unsigned long long bad = NO_KEY;
     for (int x = x0; x < x1; ++x) {
         const unsigned cb = plane_bytes(x) + rcb;   // the own cell of plane x
@@ -1305,7 +1333,10 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(eta_rows_wa
         if (AHEAD) {
 #pragma unroll
             for (int g = 0; g < NG; ++g) nx_[g] = buf_load<T>(rin, nb, (unsigned)g * fs);
-            if (x + 1 < x1) load_halo(xq);
+            // also after the segment's last plane (plane xq exists; the rows
+            // are cached and unused): a branch around these loads would make
+            // the waits for the loads above wait for them as well
+            load_halo(xq);
         }
         __syncthreads();
         if (!AHEAD) ec = eta_cell_coefs<PREC>(P, B, cb, cur_i, nxt_i, first_pass);
