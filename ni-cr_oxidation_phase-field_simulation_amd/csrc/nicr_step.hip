@@ -1294,9 +1294,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(eta_rows_wa
     // next plane's interior: the Y-neighbour blocks load the same rows as their
     // interior at that moment, so the request finds the line in L2 (or merges
     // with the neighbour's miss) instead of fetching it from HBM a plane later.
-    // In binary64 they are loaded in their plane: loading them ahead as well
-    // (114 VGPRs, still two resident blocks per CU) measured 2 % slower.
-    constexpr bool AHEAD = PREC == PREC_MIXED;
+    // Binary64 passes of up to 8 fields do the same (about 114 VGPRs, still two
+    // resident blocks per CU; 2 % faster than loading them in their plane).
+    constexpr bool AHEAD = PREC == PREC_MIXED || NG <= 8;
     T hv[HC];
     auto load_halo = [&](int X) {
         const unsigned vo = plane_bytes(X) + hrow_b;
