@@ -45,12 +45,40 @@ struct BBufs {
 
 constexpr int BW = 64;   // tile width along y (one wave)
 
+// The tile list, the instance table and each instance's buffer index and
+// failure flag are uniform over a block and do not change during k_batch_pde
+// (only k_batch_commit writes them). They are read through the scalar data
+// cache (constant address space: s_load). Vector loads of them would share the
+// in-order vmcnt counter with the state loads and stores, so every wait for a
+// metadata word would also wait for the next tile's prefetch or the previous
+// tile's stores.
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+// Records field by field: a struct copy becomes a vector load (an aggregate
+// copy is not kept in the constant address space, and there is no three-dword
+// scalar load).
+__device__ __forceinline__ BTile ld_tile(const BTile* p) {
+    return BTile{ld_uniform(&p->inst), ld_uniform(&p->x0), ld_uniform(&p->y0)};
+}
+__device__ __forceinline__ BInst ld_inst(const BInst* p) {
+    BInst I;
+    I.nx = ld_uniform(&p->nx);
+    I.ny = ld_uniform(&p->ny);
+    I.off = ld_uniform(&p->off);
+    I.tile0 = ld_uniform(&p->tile0);
+    I.ntile = ld_uniform(&p->ntile);
+    I.layer_o = ld_uniform(&p->layer_o);
+    return I;
+}
+
 template <int TX>
 struct BatchLds {
     static constexpr int W2 = BW + 4, H2 = TX + 4, W1 = BW + 2, H1 = TX + 2;
     static constexpr int P2 = W2 * H2, P1 = W1 * H1;
     static constexpr size_t bytes() {
-        return size_t(P1) * sizeof(MuMob<double>) + size_t(2) * P2 * 8 + size_t(2) * P1 * 8;
+        return size_t(P1) * sizeof(MuMob<double>) + size_t(2) * P2 * 8 + size_t(2) * P1 * 8 + size_t(W2) * 8;
     }
 };
 
@@ -65,6 +93,7 @@ __global__ __launch_bounds__(64 * TX) void k_batch_pde(Par P, BBufs B, int ntile
     double* sc = so + L::P2;                                     // [P2] cr
     double* sp = sc + L::P2;                                     // [P1] phi
     double* sg = sp + L::P1;                                     // [P1] gb
+    double* so0 = sg + L::P1;                                    // [W2] o of row x = 0 as stored
     const int tid = threadIdx.x;
     const int tx = tid >> 6, ty = tid & 63;
     // A block takes a contiguous run of tiles (the list is grouped by
@@ -87,45 +116,56 @@ __global__ __launch_bounds__(64 * TX) void k_batch_pde(Par P, BBufs B, int ntile
     constexpr int R2 = (L::P2 + NTH - 1) / NTH, R1 = (L::P1 + NTH - 1) / NTH;
     double st_o[R2], st_c[R2], st_p[R1], st_g[R1];
     auto fetch = [&](int t) {
-        const BTile tile = B.tiles[t];
-        const BInst I = B.inst[tile.inst];
-        const DevCtl* ctl = B.ctl + tile.inst;
-        const int cur = ctl->cur;
+        const BTile tile = ld_tile(B.tiles + t);
+        const BInst I = ld_inst(B.inst + tile.inst);
+        const int cur = ld_uniform(&B.ctl[tile.inst].cur);
         const double* o = B.o[cur] + I.off;
         const double* cr = B.cr[cur] + I.off;
         const double* p = B.p[cur] + I.off;
         const double* gb = B.gb + I.off;
         const int nx = I.nx, ny = I.ny;
-        const bool enf = P.enforce_layer != 0;
         auto clampx = [&](int v) { return v < 0 ? 0 : (v >= nx ? nx - 1 : v); };
         auto wrapy = [&](int v) { return ((v % ny) + ny) % ny; };
         // o, cr over rows x0-2 .. x0+TX+1 (clamped: the X ghosts are copies),
-        // columns y0-2 .. y0+65 (periodic); phi, gb over the one-cell halo
+        // columns y0-2 .. y0+65 (periodic); phi, gb over the one-cell halo.
+        // Every lane loads (slots past the tile repeat its last cell; the
+        // staging skips them) and the inlet value replaces O at the staging:
+        // with no branch around the loads and no value written into their
+        // registers, the compiler's waits do not serialise them.
 #pragma unroll
         for (int k = 0; k < R2; ++k) {
-            const int q = tid + k * NTH;
-            if (q < L::P2) {
-                const int r = q / L::W2, cy = q - r * L::W2;
-                const int xs = clampx(tile.x0 - 2 + r);
-                const long long g = (long long)xs * ny + wrapy(tile.y0 - 2 + cy);
-                st_o[k] = (enf && xs == 0) ? I.layer_o : o[g];
-                st_c[k] = cr[g];
-            }
+            const int q = min(tid + k * NTH, L::P2 - 1);
+            const int r = q / L::W2, cy = q - r * L::W2;
+            const long long g = (long long)clampx(tile.x0 - 2 + r) * ny + wrapy(tile.y0 - 2 + cy);
+            st_o[k] = o[g];
+            st_c[k] = cr[g];
         }
 #pragma unroll
         for (int k = 0; k < R1; ++k) {
-            const int q = tid + k * NTH;
-            if (q < L::P1) {
-                const int r = q / L::W1, cy = q - r * L::W1;
-                const long long g = (long long)clampx(tile.x0 - 1 + r) * ny + wrapy(tile.y0 - 1 + cy);
-                st_p[k] = p[g];
-                st_g[k] = gb[g];
-            }
+            const int q = min(tid + k * NTH, L::P1 - 1);
+            const int r = q / L::W1, cy = q - r * L::W1;
+            const long long g = (long long)clampx(tile.x0 - 1 + r) * ny + wrapy(tile.y0 - 1 + cy);
+            st_p[k] = p[g];
+            st_g[k] = gb[g];
         }
+    };
+    // A tile's results wait in registers until the next tile's values are
+    // staged: stored before the staging, they would enter the in-order vmcnt
+    // count ahead of its last wait, which would then wait for them as well.
+    double r_o = 0., r_c = 0., r_p = 0.;
+    long long r_g = -1;   // the cell in the ensemble arrays; -1: nothing to store
+    int r_nxt = 0;
+    auto store_results = [&]() {
+        if (r_g >= 0) {
+            st_nt(&B.o[r_nxt][r_g], r_o);
+            st_nt(&B.cr[r_nxt][r_g], r_c);
+            st_nt(&B.p[r_nxt][r_g], r_p);
+        }
+        r_g = -1;
     };
     if (tbeg < tend) fetch(tbeg);
     for (int t = tbeg; t < tend; ++t) {
-        const BTile tile = B.tiles[t];
+        const BTile tile = ld_tile(B.tiles + t);
         if (tile.inst != run_inst) {
             flush();
             run_inst = tile.inst;
@@ -133,12 +173,18 @@ __global__ __launch_bounds__(64 * TX) void k_batch_pde(Par P, BBufs B, int ntile
         } else if (tid < NPART) {
             B.partials[(long long)t * NPART + tid] = 0.;
         }
-        // this tile's staged values into LDS, then the next tile's loads
+        // this tile's staged values into LDS (O of rows at or before the
+        // inlet plane x = 0 is the inlet value), then the next tile's loads
+        const bool enf = P.enforce_layer != 0;
+        const BInst I = ld_inst(B.inst + tile.inst);
 #pragma unroll
         for (int k = 0; k < R2; ++k) {
             const int q = tid + k * NTH;
             if (q < L::P2) {
-                so[q] = st_o[k];
+                const int r = q / L::W2, v = tile.x0 - 2 + r;
+                const int xs = max(0, min(v, I.nx - 1));   // the clamped row
+                so[q] = (enf && xs == 0) ? I.layer_o : st_o[k];
+                if (v == 0) so0[q - r * L::W2] = st_o[k];
                 sc[q] = st_c[k];
             }
         }
@@ -150,16 +196,14 @@ __global__ __launch_bounds__(64 * TX) void k_batch_pde(Par P, BBufs B, int ntile
                 sg[q] = st_g[k];
             }
         }
+        store_results();
         __syncthreads();
         if (t + 1 < tend) fetch(t + 1);
-        const BInst I = B.inst[tile.inst];
-        DevCtl* ctl = B.ctl + tile.inst;
-        if (ctl->failed) {
+        if (ld_uniform(&B.ctl[tile.inst].failed)) {
             __syncthreads();   // the LDS tile is rewritten by the next iteration
             continue;
         }
-        const int cur = ctl->cur, nxt = cur ^ 1;
-        const double* o = B.o[cur] + I.off;
+        const int cur = ld_uniform(&B.ctl[tile.inst].cur), nxt = cur ^ 1;
         const int nx = I.nx, ny = I.ny;
         auto clampx = [&](int v) { return v < 0 ? 0 : (v >= nx ? nx - 1 : v); };
         // mu and M over the tile and its halo. A halo row outside the mesh is
@@ -205,14 +249,19 @@ __global__ __launch_bounds__(64 * TX) void k_batch_pde(Par P, BBufs B, int ntile
             p7[SYP] = sp[c1 + 1];
             const double oo = so[c2], cc = sc[c2];
             const long long c = (long long)x * ny + y;
-            const double o_stored = x == 0 ? o[c] : oo;
+            const double o_stored = x == 0 ? so0[ty + 2] : oo;
             double onv, crv, pnv, left_flux;
             trial_mm<2, PREC_FP64, double>(P, m, p7, oo, cc, x, 0., sg[c1], 0., onv, crv, pnv, left_flux);
-            finish_cell(P, c, x, o_stored, oo, cc, p7[SC], onv, crv, pnv, left_flux, acc, B.o[nxt] + I.off,
-                        B.cr[nxt] + I.off, B.p[nxt] + I.off, I.layer_o);
+            finish_cell_vals<PREC_FP64>(P, c, x, o_stored, oo, cc, p7[SC], onv, crv, pnv, left_flux, acc, I.layer_o);
+            r_o = onv;
+            r_c = crv;
+            r_p = pnv;
+            r_g = I.off + c;
         }
+        r_nxt = nxt;
         __syncthreads();   // LDS is reused by the next tile
     }
+    store_results();
     flush();
 }
 
