@@ -115,7 +115,7 @@ typedef struct nicr_sur_options {
     int32_t conserve_mass;    /* O and Cr totals after a jump follow its sources (see nicr_sur_advance) */
     double residual_tol;      /* accept when ||R|| <= residual_tol * ||proposal - x0|| for O and Cr */
     double min_value, max_value;   /* accept only O, Cr, phi inside [min, max] */
-    int32_t settle_steps;     /* extra plain steps before a proposal that follows other steps (0: none) */
+    int32_t settle_steps;     /* extra plain steps before every proposal (0: none) */
     int32_t fallback_steps;   /* plain steps after a rejected jump before the next proposal (0: jump) */
     int32_t min_jump;         /* shortest jump at the end of a call (0: full jumps only; see nicr_sur_advance) */
 } nicr_sur_options;
@@ -136,9 +136,10 @@ typedef struct nicr_sur_report {
 
 /* Advance a binary64 3D model by nsteps with network jumps of `jump` steps.
  * Each cycle starts from two consecutive states x_-1 -> x0 (update u0 =
- * x0 - x_-1): the first cycle takes one plain step for them; later cycles use
- * the last step taken (the check step of an accepted jump or the last plain
- * step of a rejected one) after settle_steps more plain steps; the proposal
+ * x0 - x_-1): the first cycle takes 1 + settle_steps plain steps for them;
+ * later cycles use the last step taken (the check step of an accepted jump or
+ * the last plain step of a rejected one) after settle_steps more plain steps;
+ * the proposal
  * xh = x0 + jump u0 + correction(x0, u0) of O, Cr, phi for the state `jump`
  * steps later (jump must equal the network's); evolving eta fields jump to
  * x0 + jump u0 with their mask rebuilt, static ones are unchanged; optionally

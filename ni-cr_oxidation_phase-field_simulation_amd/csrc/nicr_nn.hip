@@ -1798,10 +1798,14 @@ static int nicr_sur_advance_impl(void* h, void* model, int nsteps, const nicr_su
     const bool trace = tr && tr[0] == '1';
     // the update u0 of a jump is the difference of the two state buffers, so
     // they must hold consecutive states: true after the check step of an
-    // accepted jump and after plain steps, not on entry
+    // accepted jump and after plain steps, not on entry. On entry the first
+    // cycle takes one plain step for them plus the settle steps: the state
+    // handed in may itself follow a jump's check step (a run loop ends each
+    // call at a Ksp scan, often right after a jump), and a proposal made one
+    // step after a check step takes that step's transient into u0.
     bool consecutive = false;
     while (left > 0 && rep->status == NICR_STEP_OK) {
-        const int pre = consecutive ? opt->settle_steps : 1;
+        const int pre = consecutive ? opt->settle_steps : 1 + opt->settle_steps;
         // this cycle's jump: a full one, or a last shorter one of the steps left
         int Jc = J;
         if (left < J + 1 + pre) {

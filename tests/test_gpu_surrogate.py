@@ -379,10 +379,10 @@ def test_rejected_jumps_equal_plain_steps(nicr, n_eta, settle, fallback):
     rep = net.advance(a, 130, jump=20, residual_tol=1e-12, settle_steps=settle, fallback_steps=fallback)
     assert rep.status == 0 and rep.steps_done == 130
     assert rep.jumps_accepted == 0
-    # attempts: the first after one plain step, then one per (fallback + settle)
-    # steps while at least jump + 1 + settle steps are left
+    # attempts: the first after 1 + settle plain steps, then one per (fallback
+    # + settle) steps while at least jump + 1 + settle steps are left
     fb = fallback or 20
-    left, attempts, pre = 130, 0, 1
+    left, attempts, pre = 130, 0, 1 + settle
     while left >= 20 + 1 + pre:
         left -= pre + fb
         attempts += 1
