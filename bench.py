@@ -138,6 +138,82 @@ def cpu_baseline_reference(n_eta):
                       f"{dt * 1e3:.1f} ms per step"}
 
 
+C1_STEPS = 1000
+C1_SHAPE = (128, 128)
+
+
+def c1_config(nicr):
+    """Config C1: the 2D program (mpi2d, run loop 2D:1216-1236 with
+    nucleation=center) on 128 x 128 cells, grains off, Cr 0.20 +- U(0.005) from
+    std::mt19937_64(20260909), O = 0 in the bulk, the concentration inlet, the
+    centre seed of radius 5 cells, Ck legacy, bounds legacy, default dt."""
+    return nicr.make_config(2, nx=C1_SHAPE[0], ny=C1_SHAPE[1], grains=False, bc="concentration", ck="legacy",
+                            bounds="legacy", nucleation="center", radius_grid=5.0)
+
+
+def c1_gpu(nicr):
+    """C1's 1000 binary64 steps on the GPU from the reference's initial state
+    (host noise, centre seed on the device). A first model of the same config
+    takes 10 untimed steps so that first-launch costs stay out of the timing.
+    Returns the timing, the initial o, cr, p (after the seed) and the final ones."""
+    syn = importlib.import_module(PKG_NAME + ".synthetic")
+    cfg = c1_config(nicr)
+    o, cr, p = syn.noisy_alloy(C1_SHAPE, 0.20, 0.005, seed=20260909)
+    models = []
+    for _ in range(2):
+        m = nicr.Model(cfg)
+        for name, a in (("o", o), ("cr", cr), ("p", p)):
+            m.set(name, a)
+        seeded = m.initialize_center_oxide()
+        models.append(m)
+    models[0].step(10)
+    models[0].sync()
+    m = models[1]
+    initial = {f: m.get(f) for f in ("o", "cr", "p")}
+    m.sync()
+    t0 = time.perf_counter()
+    r = m.try_step(C1_STEPS)
+    m.sync()
+    t1 = time.perf_counter()
+    if r.status != 0 or r.steps_done != C1_STEPS:
+        raise RuntimeError(f"C1: step rejected (status {r.status})")
+    fields = {f: m.get(f) for f in ("o", "cr", "p")}
+    for mm in models:
+        mm.close()
+    cells = C1_SHAPE[0] * C1_SHAPE[1]
+    return {"seconds": round(t1 - t0, 4), "ms_per_step": round((t1 - t0) / C1_STEPS * 1e3, 4),
+            "value": round(cells * C1_STEPS / (t1 - t0) / 1e9, 4), "unit": "Gcell-steps/s",
+            "centre_seed_cells": seeded}, initial, fields
+
+
+def cpu_baseline_c1(nicr, initial, gpu_fields):
+    """The reference's own 2D Model (oracle/_ref/libref2d.so, compiled from
+    mpi2d/NiCr2D_grains.cpp) on config C1 on one host thread, as mpirun -np 1
+    runs it: constructed with C1's config (inlet layer, centre seed), its o, cr,
+    p replaced by the GPU run's initial state (its constructor has no Cr noise:
+    bulk Cr 0.1986), then 1000 Model::step calls are timed. Also whether its
+    final o, cr, p equal the GPU's bit for bit. None when it was not built."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import binding
+    if not binding.ref_available(2):
+        return None
+    ref = binding.RefModel(c1_config(nicr))
+    for f, a in initial.items():
+        ref.set(f, a)
+    t0 = time.perf_counter()
+    done = ref.run(C1_STEPS)
+    t1 = time.perf_counter()
+    if done != C1_STEPS:
+        raise RuntimeError(f"C1 reference run stopped after {done} steps")
+    equal = all(np.array_equal(ref.get(f), gpu_fields[f]) for f in ("o", "cr", "p"))
+    cells = C1_SHAPE[0] * C1_SHAPE[1]
+    return {"value": round(cells * C1_STEPS / (t1 - t0) / 1e9, 6), "unit": "Gcell-steps/s", "cores": 1,
+            "kind": "reference", "seconds": round(t1 - t0, 3), "ms_per_step": round((t1 - t0) / C1_STEPS * 1e3, 4),
+            "sample": f"mpi2d Model (oracle/_ref/libref2d.so) on one host thread: the whole C1 run, "
+                      f"{C1_SHAPE[0]}x{C1_SHAPE[1]} cells, {C1_STEPS} steps",
+            "fields_bitwise_equal_gpu": bool(equal)}
+
+
 def cpu_baseline(nicr, n_eta, seed):
     """The CPU oracle (one thread) on a 64 x 256 x 256 slab of the same workload, 12 steps."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -543,6 +619,10 @@ def main():
         "mixed_error": mixed_error,
         "copy_probe_GBs": round(copy_gbs, 1),
     }
+    c1, c1_initial, c1_fields = c1_gpu(nicr)
+    out["c1"] = {"workload": f"2D {C1_SHAPE[0]}x{C1_SHAPE[1]} single crystal, centre seed r=5 (80 cells), Cr "
+                             "0.20+-0.005 (mt19937_64), O inlet on x=0 (concentration BC), Ck legacy, bounds legacy, "
+                             f"{C1_STEPS} binary64 steps", "gpu": c1}
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(nicr, 64, 200, 20260909)
     if not args.no_surrogate:
@@ -550,6 +630,10 @@ def main():
     if not args.no_cpu:
         ref = cpu_baseline_reference(args.eta)
         out["cpu_baseline"] = ref if ref is not None else cpu_baseline(nicr, args.eta, 20260909)
+        c1_ref = cpu_baseline_c1(nicr, c1_initial, c1_fields)
+        if c1_ref is not None:
+            out["c1"]["cpu_reference"] = c1_ref
+            out["c1"]["speedup_vs_reference"] = round(c1["value"] / c1_ref["value"], 1)
     print(json.dumps(out), flush=True)
 
 
