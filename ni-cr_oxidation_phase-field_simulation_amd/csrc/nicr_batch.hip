@@ -285,24 +285,27 @@ __global__ __launch_bounds__(256) void k_batch_commit(BBufs B, int dry) {
         return;
     }
     const BInst I = B.inst[i];
-    __shared__ DD sh[16];
+    __shared__ DD sh[6 * 4];
     DD acc[6];
     for (int s = 0; s < 6; ++s) acc[s] = DD{0., 0.};
     for (int b = threadIdx.x; b < I.ntile; b += blockDim.x) {
         const double* w = B.partials + (long long)(I.tile0 + b) * NPART;
         for (int s = 0; s < 6; ++s) acc[s] = dd_add(acc[s], DD{w[2 * s], w[2 * s + 1]});
     }
-    DD r[6];
-    for (int s = 0; s < 6; ++s) r[s] = block_reduce_dd(acc[s], sh);
-    if (threadIdx.x == 0) {
-        const int map[5] = {6, 2, 3, 4, 5};   // bcO, ckO, ckCr, clipO, clipCr
-        for (int s = 0; s < 5; ++s) {
-            DD v{ctl->budget[2 * map[s]], ctl->budget[2 * map[s] + 1]};
-            v = dd_add(v, r[s]);
-            ctl->budget[2 * map[s]] = v.hi;
-            ctl->budget[2 * map[s] + 1] = v.lo;
-        }
-        ctl->clipped += (long long)r[5].hi;
+    const DD r = block_reduce_dd6(acc, sh);
+    // thread s < 5 adds sum s (bcO, ckO, ckCr, clipO, clipCr) into its budget
+    // slot (6, 2, 3, 4, 5), thread 5 the clipped count
+    const int s = threadIdx.x;
+    if (s < 5) {
+        const int k = s == 0 ? 6 : s + 1;
+        DD v{ctl->budget[2 * k], ctl->budget[2 * k + 1]};
+        v = dd_add(v, r);
+        ctl->budget[2 * k] = v.hi;
+        ctl->budget[2 * k + 1] = v.lo;
+    } else if (s == 5) {
+        ctl->clipped += (long long)r.hi;
+    }
+    if (s == 0) {
         ctl->cur ^= 1;
         ctl->steps_done += 1;
     }

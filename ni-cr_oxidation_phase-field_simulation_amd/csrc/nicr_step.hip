@@ -202,6 +202,38 @@ __device__ __forceinline__ DD block_reduce_dd(DD v, DD* sh) {
     return r;
 }
 
+// Six double-double block sums at once. Each value goes through the same
+// operations in the same order as in block_reduce_dd (the shuffle tree, then
+// the waves in order from a zero sum), so the results are bitwise those of six
+// block_reduce_dd calls; the six chains interleave, one barrier pair serves
+// all of them, and thread s (s < 6) sums value s across the waves. sh holds
+// 6 * (blockDim.x / 64) entries. Returns value threadIdx.x on threads 0..5.
+__device__ __forceinline__ DD block_reduce_dd6(const DD (&in)[6], DD* sh) {
+    DD v[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) v[s] = in[s];
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            DD o;
+            o.hi = __shfl_down(v[s].hi, off);
+            o.lo = __shfl_down(v[s].lo, off);
+            v[s] = dd_add(v[s], o);
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+    __syncthreads();   // sh may still be read by a previous reduction
+    if (lane == 0) {
+#pragma unroll
+        for (int s = 0; s < 6; ++s) sh[s * nw + wave] = v[s];
+    }
+    __syncthreads();
+    DD r{0., 0.};
+    if (threadIdx.x < 6)
+        for (int w = 0; w < nw; ++w) r = dd_add(r, sh[threadIdx.x * nw + w]);
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // k_trial: mobilities + conservative six-face update of O and Cr, Allen-Cahn
 // phi update, legacy Ck, finite check and bounds (2D:430-433,435-450,553-591;
@@ -442,16 +474,13 @@ __device__ __forceinline__ void finish_cell(const Par& P, long long c, int x, do
 // the failing cell into *key.
 __device__ __forceinline__ void block_partials(const StepAcc& a, double* w, unsigned long long* key) {
     if (a.bad != NO_KEY) atomicMin(key, a.bad);
-    __shared__ DD sh[16];
-    const DD r0 = block_reduce_dd(a.bc, sh);
-    const DD r1 = block_reduce_dd(a.ckO, sh);
-    const DD r2 = block_reduce_dd(a.ckCr, sh);
-    const DD r3 = block_reduce_dd(a.clO, sh);
-    const DD r4 = block_reduce_dd(a.clCr, sh);
-    const DD r5 = block_reduce_dd(DD{a.n_clip, 0.}, sh);
-    if (threadIdx.x == 0) {
-        w[0] = r0.hi; w[1] = r0.lo; w[2] = r1.hi; w[3] = r1.lo; w[4] = r2.hi; w[5] = r2.lo;
-        w[6] = r3.hi; w[7] = r3.lo; w[8] = r4.hi; w[9] = r4.lo; w[10] = r5.hi; w[11] = 0.;
+    __shared__ DD sh[6 * 16];
+    const DD v[6] = {a.bc, a.ckO, a.ckCr, a.clO, a.clCr, DD{a.n_clip, 0.}};
+    const DD r = block_reduce_dd6(v, sh);
+    // slots: bcO, ckO, ckCr, clipO, clipCr as (hi, lo), then the clipped count and 0
+    if (threadIdx.x < 6) {
+        w[2 * threadIdx.x] = r.hi;
+        w[2 * threadIdx.x + 1] = threadIdx.x < 5 ? r.lo : 0.;
     }
 }
 // ... into this block's partial slot.
@@ -813,26 +842,28 @@ __device__ __forceinline__ void commit_body(const double* partials, DevCtl* ctl,
         }
         return;
     }
-    __shared__ DD sh[16];
+    __shared__ DD sh[6 * 16];
     DD acc[6];
     for (int s = 0; s < 6; ++s) acc[s] = DD{0., 0.};
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
         const double* w = partials + (long long)b * NPART;
         for (int s = 0; s < 6; ++s) acc[s] = dd_add(acc[s], DD{w[2 * s], w[2 * s + 1]});
     }
-    DD r[6];
-    for (int s = 0; s < 6; ++s) r[s] = block_reduce_dd(acc[s], sh);
-    if (threadIdx.x == 0) {
-        // partial slots: bcO, ckO, ckCr, clipO, clipCr, clipped
-        // budget slots:  nucO0 nucCr1 ckO2 ckCr3 clipO4 clipCr5 bcO6 bcCr7
-        const int map[5] = {6, 2, 3, 4, 5};
-        for (int s = 0; s < 5; ++s) {
-            DD cur{ctl->budget[2 * map[s]], ctl->budget[2 * map[s] + 1]};
-            cur = dd_add(cur, r[s]);
-            ctl->budget[2 * map[s]] = cur.hi;
-            ctl->budget[2 * map[s] + 1] = cur.lo;
-        }
-        ctl->clipped += (long long)r[5].hi;
+    const DD r = block_reduce_dd6(acc, sh);
+    // partial slots: bcO, ckO, ckCr, clipO, clipCr, clipped
+    // budget slots:  nucO0 nucCr1 ckO2 ckCr3 clipO4 clipCr5 bcO6 bcCr7
+    // thread s < 5 adds sum s into its budget slot, thread 5 the clipped count
+    const int s = threadIdx.x;
+    if (s < 5) {
+        const int k = s == 0 ? 6 : s + 1;
+        DD cur{ctl->budget[2 * k], ctl->budget[2 * k + 1]};
+        cur = dd_add(cur, r);
+        ctl->budget[2 * k] = cur.hi;
+        ctl->budget[2 * k + 1] = cur.lo;
+    } else if (s == 5) {
+        ctl->clipped += (long long)r.hi;
+    }
+    if (s == 0) {
         ctl->cur ^= 1;
         ctl->steps_done += 1;
     }
