@@ -92,6 +92,13 @@ struct Ctx {
     int eta_nseg = 0;              // X segments per column (NICR_ETA_NSEG; 0: one wave of resident blocks)
     int eta_ng = 16;               // max eta fields per marching pass (NICR_ETA_NG: 8, 16, 32)
     bool eta_rows_disabled = false;  // NICR_ETA_ROWS=0 keeps the 64-wide tiles where whole rows would fit
+    // 2D binary64 step through the ensemble kernels (batch2d_step, set up on
+    // the first step); NICR_BATCH2D=0 keeps k_mu + k_trial
+    bool b1_disabled = false;
+    void* b1_inst = nullptr;      // one-instance table (BInst)
+    void* b1_tiles = nullptr;     // its tile list (BTile)
+    double* b1_partials = nullptr;
+    int b1_ntile = 0, b1_tx = 0;
     std::string launch_error;        // set-up failure of a marching kernel (reported by nicr_step)
     size_t bytes = 0;
     std::vector<void*> allocs;
@@ -353,8 +360,12 @@ int launch_pde_march(Ctx* c, const Bufs<TE, TM>& B) {
     }
 }
 
+bool batch2d_step(Ctx* c);   // nicr_batch.hip
 template <int DIM, int PREC, class TE, class TM>
 void launch_step(Ctx* c) {
+    if constexpr (DIM == 2 && PREC == PREC_FP64) {
+        if (batch2d_step(c)) return;
+    }
     auto B = bufs<TE, TM>(c);
     int nparts = -1;
     if constexpr (DIM == 3) {
@@ -597,6 +608,7 @@ int nicr_create(const nicr_config* cfg, void** out) {
     if (const char* ng = std::getenv("NICR_ETA_NG")) c->eta_ng = std::atoi(ng);
     if (const char* t = std::getenv("NICR_PDE_TY")) c->pde_ty = std::atoi(t);
     if (const char* t = std::getenv("NICR_ETA_ROWS")) c->eta_rows_disabled = std::atoi(t) == 0;
+    if (const char* t = std::getenv("NICR_BATCH2D")) c->b1_disabled = std::atoi(t) == 0;
     if (const char* t = std::getenv("NICR_PDE_NSEG")) c->pde_nseg = std::atoi(t);
     if (c->cfg.dim == 2) c->cfg.nz = 1;
     if (!(c->cfg.grains && c->cfg.grain_evolution)) c->cfg.n_eta = 0;

@@ -191,3 +191,72 @@ def test_benchmark_ensemble_bitwise_vs_oracle(nicr, oracle):
         bo, bg = ora.b, ens.budget(i)
         for k, v in bo.items():
             assert abs(bg[k] - v) <= BUDGET_RTOL * (1 + abs(v)), f"instance {i}: budget {k}"
+
+
+def _context(nicr, cfg, batch2d):
+    """A 2D context created with NICR_BATCH2D on (the ensemble kernels step it)
+    or off (k_mu + k_trial)."""
+    import os
+    old = os.environ.get("NICR_BATCH2D")
+    os.environ["NICR_BATCH2D"] = "1" if batch2d else "0"
+    try:
+        return nicr.Model(cfg)
+    finally:
+        if old is None:
+            os.environ.pop("NICR_BATCH2D", None)
+        else:
+            os.environ["NICR_BATCH2D"] = old
+
+
+@pytest.mark.parametrize("bc", ["concentration", "mu", "closed"])
+@pytest.mark.parametrize("grains", [False, True])
+@pytest.mark.parametrize("shape,oxygen", [((40, 70), True), ((7, 130), True), ((33, 20), False)])
+def test_single_2d_context_through_ensemble_kernels(nicr, bc, grains, shape, oxygen):
+    """A 2D binary64 context without eta fields steps through the ensemble
+    kernels (one launch fewer per step). Fields must be bitwise those of the
+    k_mu + k_trial path, budgets within 1e-13 (the partials are grouped
+    differently), clipped counts equal, over 40 steps and after a rejection."""
+    st = _inst_state(shape, 0.18, REF_LAYER_O, seed=5)
+    cfg = nicr.make_config(2, nx=shape[0], ny=shape[1], grains=grains, grain_evolution=False, bc=bc,
+                           dt=1e-12, bounds="legacy", ck="legacy", oxygen_diffusion=oxygen)
+    a, b = _context(nicr, cfg, True), _context(nicr, cfg, False)
+    try:
+        for m in (a, b):
+            for name, arr in zip(("o", "cr", "p"), st[:3]):
+                m.set(name, arr)
+            if grains:
+                m.set("gb", st[3])
+        ra, rb = a.try_step(40), b.try_step(40)
+        assert ra.status == rb.status == 0 and ra.steps_done == rb.steps_done == 40
+        for f in ("o", "cr", "p"):
+            assert np.array_equal(a.get(f), b.get(f)), f"{bc} grains={grains} {shape}: {f}"
+        ba, bb = a.b, b.b
+        for k, v in bb.items():
+            assert abs(ba[k] - v) <= 1e-13 * (1 + abs(v)), f"budget {k}: {ba[k]!r} vs {v!r}"
+        assert ba["clipped"] == bb["clipped"]
+    finally:
+        a.close()
+        b.close()
+
+
+def test_single_2d_context_rejection_matches(nicr):
+    """bounds=stop on a state outside the bounds: both 2D paths reject the
+    first step with the same report and leave the same state."""
+    shape = (16, 24)
+    o, cr, p, _ = _inst_state(shape, 0.18, REF_LAYER_O, seed=7)
+    cr[5, 9] = 1.2
+    cfg = nicr.make_config(2, nx=shape[0], ny=shape[1], grains=False, bc="concentration", dt=1e-12,
+                           bounds="stop", ck="legacy")
+    a, b = _context(nicr, cfg, True), _context(nicr, cfg, False)
+    try:
+        for m in (a, b):
+            for name, arr in zip(("o", "cr", "p"), (o, cr, p)):
+                m.set(name, arr)
+        ra, rb = a.try_step(3), b.try_step(3)
+        assert ra.status == rb.status != 0 and ra.steps_done == rb.steps_done == 0
+        assert (ra.i, ra.j, ra.o, ra.cr, ra.phi) == (rb.i, rb.j, rb.o, rb.cr, rb.phi)
+        for f in ("o", "cr", "p"):
+            assert np.array_equal(a.get(f), b.get(f)), f
+    finally:
+        a.close()
+        b.close()
