@@ -210,7 +210,8 @@ def _context(nicr, cfg, batch2d):
 
 @pytest.mark.parametrize("bc", ["concentration", "mu", "closed"])
 @pytest.mark.parametrize("grains", [False, True])
-@pytest.mark.parametrize("shape,oxygen", [((40, 70), True), ((7, 130), True), ((33, 20), False)])
+@pytest.mark.parametrize("shape,oxygen", [((40, 70), True), ((7, 130), True), ((33, 20), False), ((2, 9), True),
+                                          ((4, 3), True)])
 def test_single_2d_context_through_ensemble_kernels(nicr, bc, grains, shape, oxygen):
     """A 2D binary64 context without eta fields steps through the ensemble
     kernels (one launch fewer per step). Fields must be bitwise those of the
