@@ -609,7 +609,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": eta_kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes_per_launch": int(eta_bytes), "ms_per_launch": round(eta_ms, 5),
-                     "traffic": traffic},
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_detail": traffic},
         "roofline_pde": {"bound": "hbm", "kernel": "k_pde_march",
                          "achieved": round(pde_bytes / (pde_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(pde_bytes / (pde_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
