@@ -730,6 +730,7 @@ Geo geo(const Net* n, int nb, int level, int cin, int cout, int taps) {
 struct HaloCfg {
     int ncu = 0;
     bool enabled = true;
+    bool half_k = true;   // NICR_CONV_HALFK=0: the input layer computes all 32 channels
     bool attr_set = false;
 };
 static HaloCfg g_halo;
@@ -743,9 +744,15 @@ static bool halo_ready() {
         g_halo.ncu = ncu;
         const char* e = getenv("NICR_CONV_HALO");
         g_halo.enabled = !(e && e[0] == '0');
-        if (hipFuncSetAttribute((const void*)k_conv_halo<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        const char* hk = getenv("NICR_CONV_HALFK");
+        g_halo.half_k = !(hk && hk[0] == '0');
+        if (hipFuncSetAttribute((const void*)k_conv_halo<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 HC_LDS_BYTES) != hipSuccess ||
-            hipFuncSetAttribute((const void*)k_conv_halo<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+            hipFuncSetAttribute((const void*)k_conv_halo<false, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                HC_LDS_BYTES) != hipSuccess ||
+            hipFuncSetAttribute((const void*)k_conv_halo<true, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                HC_LDS_BYTES) != hipSuccess ||
+            hipFuncSetAttribute((const void*)k_conv_halo<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 HC_LDS_BYTES) != hipSuccess)
             g_halo.enabled = false;
         g_halo.attr_set = true;
@@ -755,20 +762,28 @@ static bool halo_ready() {
 
 bool halo_applies(const Geo& g) { return g.s0 % HT_X == 0 && g.s1 % HT_Y == 0 && g.s2 % HT_Z == 0 && halo_ready(); }
 
-// Xc: coarse source of channels 0 .. g.cu - 1 (halo kernel only)
+// Xc: coarse source of channels 0 .. g.cu - 1 (halo kernel only). half_k: X
+// has 32 channels of which 16..31 are zero (the network's input tensor).
 void launch_conv(Net* n, const Geo& g, const bf16* X, const bf16* W, const float* bias, bf16* Y, bool relu,
-                 const bf16* Xc = nullptr) {
+                 const bf16* Xc = nullptr, bool half_k = false) {
     if (halo_applies(g)) {
         const int units = g.nb * (g.s0 / HT_X) * (g.s1 / HT_Y) * (g.s2 / HT_Z) * (g.cout / 32);
         const int nblock = std::min(units, g_halo.ncu);
         const int upb = (units + nblock - 1) / nblock;
         const int grid = (units + upb - 1) / upb;
-        if (relu)
-            NN_LAUNCH(k_conv_halo<true>, dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W, bias,
-                               Y, units, upb, Xc);
+        const bool hk = half_k && g.cin == 32 && g.cu == 0 && g_halo.half_k;
+        if (relu && hk)
+            NN_LAUNCH((k_conv_halo<true, true>), dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W,
+                      bias, Y, units, upb, Xc);
+        else if (hk)
+            NN_LAUNCH((k_conv_halo<false, true>), dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W,
+                      bias, Y, units, upb, Xc);
+        else if (relu)
+            NN_LAUNCH((k_conv_halo<true, false>), dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W,
+                      bias, Y, units, upb, Xc);
         else
-            NN_LAUNCH(k_conv_halo<false>, dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W,
-                               bias, Y, units, upb, Xc);
+            NN_LAUNCH((k_conv_halo<false, false>), dim3(grid), dim3(HC_THREADS), HC_LDS_BYTES, n->stream, g, X, W,
+                      bias, Y, units, upb, Xc);
         return;
     }
     const int P = g.positions();
@@ -927,7 +942,9 @@ void forward(Net* n, int nb) {
     ConvL* c = n->conv;
     auto conv = [&](int i) {
         const ConvL& L = c[i];
-        launch_conv(n, geo(n, nb, L.level, L.cin, L.cout, 27), L.x, L.Wb, n->param + L.b_off, L.y, L.relu);
+        // layer 0 reads the input tensor, whose channels 16..31 are zero
+        launch_conv(n, geo(n, nb, L.level, L.cin, L.cout, 27), L.x, L.Wb, n->param + L.b_off, L.y, L.relu, nullptr,
+                    i == 0);
     };
     auto pool = [&](int level, int C, const bf16* X, bf16* Y) {
         const long long nq = (long long)nb * n->V(level + 1);

@@ -188,7 +188,11 @@ constexpr int HC_THREADS = 512;                 // 8 waves
 constexpr int H_LOADS = (H_ROWS * 4 + HC_THREADS - 1) / HC_THREADS;
 constexpr int HC_MT = HC_RB / (HC_THREADS / 64);  // row-blocks per wave: 4
 
-template <bool RELU>
+// HALFK: the input has one 32-channel chunk whose channels 16..31 are zero
+// (the network's input tensor), so only the first K step of each tap is
+// computed and only the lower half of each row is loaded; the skipped MFMAs
+// would add exact zeros.
+template <bool RELU, bool HALFK = false>
 __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __restrict__ X,
                                                           const bf16* __restrict__ W, const float* __restrict__ bias,
                                                           bf16* __restrict__ Y, int units, int upb,
@@ -222,7 +226,7 @@ __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __r
         for (int i = 0; i < H_LOADS; ++i) {
             const int idx = tid + i * HC_THREADS;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (idx < H_ROWS * 4) {
+            if (idx < H_ROWS * 4 && (!HALFK || (idx & 3) < 2)) {
                 const int row = idx >> 2, q = idx & 3;
                 const int hz = row % HH_Z, hy = (row / HH_Z) % HH_Y, hx = row / (HH_Z * HH_Y);
                 const int gx = x0 + hx;
@@ -266,11 +270,12 @@ __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __r
         // weights of the 3 taps of a group (c = 0, 1, 2; K steps 0, 1), loaded
         // one group ahead: the next group's fragments are in flight for the 24
         // MFMAs of the current group
+        constexpr int KS = HALFK ? 1 : 2;   // K steps of 16 channels per tap
         auto load_group = [&](int ab, uint4 (&w)[3][2]) {
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
-                for (int kk = 0; kk < 2; ++kk)
+                for (int kk = 0; kk < KS; ++kk)
                     w[c][kk] = *reinterpret_cast<const uint4*>(wl + (ab * 3 + c) * wtap + kk * 16);
         };
         uint4 bw[3][2], bn[3][2];
@@ -287,13 +292,13 @@ __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __r
 #pragma unroll
             for (int m = 0; m < HC_MT; ++m) af[0][m] = *reinterpret_cast<const uint4*>(gbase + aoff(0, 0, m));
 #pragma unroll
-            for (int st = 0; st < 6; ++st) {
-                const int c = st >> 1, kk = st & 1;
-                if (st + 1 < 6) {
+            for (int st = 0; st < 3 * KS; ++st) {
+                const int c = st / KS, kk = st % KS;
+                if (st + 1 < 3 * KS) {
 #pragma unroll
                     for (int m = 0; m < HC_MT; ++m)
                         af[(st + 1) & 1][m] =
-                            *reinterpret_cast<const uint4*>(gbase + aoff((st + 1) >> 1, (st + 1) & 1, m));
+                            *reinterpret_cast<const uint4*>(gbase + aoff((st + 1) / KS, (st + 1) % KS, m));
                 }
                 const bf16x8 bfr = as_bf16x8(bw[c][kk]);
 #pragma unroll
@@ -303,7 +308,7 @@ __global__ __launch_bounds__(HC_THREADS) void k_conv_halo(Geo g, const bf16* __r
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
-                for (int kk = 0; kk < 2; ++kk) bw[c][kk] = bn[c][kk];
+                for (int kk = 0; kk < KS; ++kk) bw[c][kk] = bn[c][kk];
         }
         if (ch == nch - 1) {
             int t = u / nco;

@@ -490,3 +490,46 @@ def test_training_reduces_loss_and_jumps_are_checked(nicr, n_eta):
     assert rep.max_residual_cr < 0.6 * rz.max_residual_cr
     zero.close()
     net.close()
+
+
+_HALFK_SCRIPT = """
+import importlib, sys
+import numpy as np
+sur = importlib.import_module("ni-cr_oxidation_phase-field_simulation_amd.surrogate")
+shape = (16, 32, 32)
+net = sur.UNet(shape, max_batch=1, train=False, seed=7)
+rng = np.random.default_rng(11)
+net.set_params(rng.normal(0, 0.03, size=net.nparam).astype(np.float32))
+net.set_norm(phi_mean=0.5, phi_scale=0.3, gb_mean=0.5, gb_scale=0.3, c_mean=[-3.5, 0.2, -7.0, -6.0],
+             c_scale=[0.5, 0.02, 0.5, 0.5], gain=[1.0, 1.0, 1.0], jump=50)
+sz = (1,) + shape
+fields = np.stack([rng.uniform(0, 2e-3, size=sz), 0.2 + 0.01 * rng.normal(size=sz), rng.uniform(0, 1, size=sz),
+                   rng.uniform(0, 1, size=sz), 1e-7 * rng.normal(size=sz), 1e-6 * rng.normal(size=sz),
+                   1e-4 * rng.normal(size=sz)], axis=1)
+np.save(sys.argv[1], net.forward(fields))
+net.close()
+"""
+
+
+def test_input_layer_half_k_equals_full_k(tmp_path):
+    """The input convolution computes only the 16 input channels that carry
+    data (the input tensor's channels 16..31 are zero): the network's output
+    must be bitwise that of the 32-channel computation (NICR_CONV_HALFK=0). The
+    switch is read once per process, so each variant runs in a child process."""
+    import os
+    import subprocess
+    import sys
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = tmp_path / "halfk.py"
+    script.write_text(_HALFK_SCRIPT)
+    outs = []
+    for flag in ("1", "0"):
+        env = dict(os.environ, NICR_CONV_HALFK=flag, PYTHONPATH=repo)
+        path = tmp_path / f"out{flag}.npy"
+        print(f"[halfk] NICR_CONV_HALFK={flag}", flush=True)
+        r = subprocess.run([sys.executable, str(script), str(path)], cwd=repo, env=env, capture_output=True,
+                           text=True, timeout=240)
+        assert r.returncode == 0, r.stderr[-2000:]
+        outs.append(np.load(path))
+    assert np.array_equal(outs[0], outs[1])
+    assert np.abs(outs[0]).max() > 0
