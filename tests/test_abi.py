@@ -82,3 +82,27 @@ def test_grain_entry_points_reject_bad_arguments_before_the_device():
     cnt = (ctypes.c_int64 * 4)()
     assert gt(None, db, ext, cnt) == -1
     assert b"null argument" in so.nicr_last_error()
+
+
+@pytest.mark.parametrize("change, message", [
+    ({"abi_version": 1}, b"ABI version mismatch"),
+    ({"dim": 4}, b"dim must be 2 or 3"),
+    ({"nx": 0}, b"mesh dimensions must be positive"),
+    ({"nz": -3}, b"mesh dimensions must be positive"),
+    ({"precision": 7}, b"unknown precision"),
+    ({"n_eta": -1}, b"bad n_eta"),
+    # 2^20 x 2^12 x 2^12 cells: more 256-cell tiles than 32-bit tile indices hold
+    ({"nx": 1 << 20, "ny": 1 << 12, "nz": 1 << 12}, b"mesh too large for 32-bit tile indexing"),
+])
+def test_create_rejects_bad_configurations_before_the_device(nicr, change, message):
+    """nicr_create validates the configuration on the host, before any HIP call
+    (the error path runs without a GPU): NICR_E_INVALID (-1), the reason in
+    nicr_last_error, and no context."""
+    lib = nicr.load_library()
+    cfg = nicr.make_config(3, nx=16, ny=16, nz=16)
+    for k, v in change.items():
+        setattr(cfg, k, v)
+    ctx = ctypes.c_void_p()
+    assert lib.nicr_create(ctypes.byref(cfg), ctypes.byref(ctx)) == -1
+    assert message in lib.nicr_last_error()
+    assert not ctx.value
