@@ -92,6 +92,7 @@ struct Ctx {
     int eta_nseg = 0;              // X segments per column (NICR_ETA_NSEG; 0: one wave of resident blocks)
     int eta_ng = 16;               // max eta fields per marching pass (NICR_ETA_NG: 8, 16, 32)
     bool eta_rows_disabled = false;  // NICR_ETA_ROWS=0 keeps the 64-wide tiles where whole rows would fit
+    bool eta_march64_fp64 = true;    // NICR_ETA_MARCH64=0: binary64 with more than 8 fields unblocked off row meshes
     // 2D binary64 step through the ensemble kernels (batch2d_step, set up on
     // the first step); NICR_BATCH2D=0 keeps k_mu + k_trial
     bool b1_disabled = false;
@@ -314,9 +315,10 @@ bool launch_eta_march(Ctx* c, const Bufs<TE, TM>& B) {
         (unsigned long long)c->P.fstride * sizeof(TE) * (unsigned long long)ng_max >= (1ull << 32))
         return false;
     if constexpr (PREC == PREC_FP64) {
-        // more than 8 fields in binary64: several passes, only with whole-row
-        // tiles (with 64-wide tiles the unblocked kernel is faster there)
-        if (n > 8 && !eta_rows_apply(c, 8, sizeof(TE))) return false;
+        // more than 8 fields in binary64: passes of 8, on whole-row or 64-wide
+        // tiles (64-wide passes are 1.9x faster than the unblocked kernel at
+        // 256x256x512 and 256x256x320 with 32 fields)
+        if (n > 8 && !c->eta_march64_fp64 && !eta_rows_apply(c, 8, sizeof(TE))) return false;
         launch_eta_march_ty<PREC, 2>(c, B, cap);
     } else {
         if (c->eta_ty == 2) launch_eta_march_ty<PREC, 2>(c, B, cap);
@@ -608,6 +610,7 @@ int nicr_create(const nicr_config* cfg, void** out) {
     if (const char* ng = std::getenv("NICR_ETA_NG")) c->eta_ng = std::atoi(ng);
     if (const char* t = std::getenv("NICR_PDE_TY")) c->pde_ty = std::atoi(t);
     if (const char* t = std::getenv("NICR_ETA_ROWS")) c->eta_rows_disabled = std::atoi(t) == 0;
+    if (const char* t = std::getenv("NICR_ETA_MARCH64")) c->eta_march64_fp64 = std::atoi(t) != 0;
     if (const char* t = std::getenv("NICR_BATCH2D")) c->b1_disabled = std::atoi(t) == 0;
     if (const char* t = std::getenv("NICR_PDE_NSEG")) c->pde_nseg = std::atoi(t);
     if (c->cfg.dim == 2) c->cfg.nz = 1;
